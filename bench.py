@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Throughput of the 2-D Euler finite-volume step on one MI355X.
+
+Workload (the headline configuration): 2-D Noh implosion, 4096 x 4096 cells on
+[0,1]^2, gamma = 5/3, rho = 1, p = 1e-6, unit inflow speed toward the origin,
+reflecting walls at x = 0 and y = 0 and outflow at the far walls; characteristic
+WENO5 (WENO5-Z) reconstruction, HLLC flux, SSP-RK3, CFL 0.4.  One "step" is one full
+RK3 step (dt reduction + 3 stages) over the grid; the unit is Mcell-updates/s
+(interior cells x steps / s), inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  The CPU baseline is the reference's own FP64 OpenMP step
+(oracle/_ref/ref_harness) on a bounded sample of the same workload, on this host's
+cores; without that binary it falls back to the oracle port.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-updates/s WENO5-HLLC 4096^2: mixed vs FP64 GPU at L1(rho)<=1e-5; %HBM"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell-update of an SSP-RK3 step with FP64 state (4 vars):
+# stage 1 reads U^n and writes U1 (64 B), stages 2 and 3 read the stage input and
+# U^n and write the output (96 B each); the dt reduction is folded in.
+STEP_BYTES_PER_CELL = 256.0
+# The x-face flux kernel reads the stage state (32 B per cell) and writes four FP64
+# face fluxes (32 B per face): 64 B per face.
+FLUX_BYTES_PER_FACE = 64.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """Minimal hipEvent_t binding (timing on a given stream)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        f = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(f), a, b) == 0
+        return float(f.value)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def run_gpu(cfg, U0, steps, warmup):
+    from overheatingproblem_amd.capi import Solver
+    ev = HipEvents()
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        for _ in range(warmup):
+            s.time_advance(s.get_delta_t())
+        faults = s.sync()
+        stream = s.stream()
+        s.set_kernel_timing(True)
+        s.kernel_times()  # reset
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.time_advance(s.get_delta_t())
+        ev.record(e1, stream)
+        gpu_ms = ev.elapsed_ms(e0, e1)
+        t1 = time.perf_counter()
+        kt = s.kernel_times()
+        faults |= s.sync()
+        U = s.get_state()
+    return {"gpu_ms": gpu_ms, "wall_s": t1 - t0, "kernels": kt, "faults": faults, "U": U}
+
+
+def cpu_baseline(n, steps, cores):
+    """Reference FP64 OpenMP step (or the oracle port) on `steps` steps of the same
+    workload at n^2."""
+    from overheatingproblem_amd import cases
+    cfg, U = cases.noh(n)
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="true")
+    if os.path.exists(harness):
+        from oracle.pyoracle import harness_job
+        with tempfile.TemporaryDirectory() as td:
+            with open(os.path.join(td, "job.txt"), "w") as f:
+                f.write(harness_job("bench", cfg, steps=steps))
+            U.astype("<f8").tofile(os.path.join(td, "in.bin"))
+            r = subprocess.run([harness, os.path.join(td, "job.txt"), os.path.join(td, "in.bin"),
+                                os.path.join(td, "out.bin")], capture_output=True, text=True,
+                               env=env, check=True)
+        rec = json.loads(r.stdout.strip().splitlines()[-1])
+        secs, kind, threads = rec["seconds"], "reference", rec["threads"]
+    else:
+        os.environ["OMP_NUM_THREADS"] = str(cores)
+        from oracle import pyoracle
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pyoracle.time_advance(cfg, U, pyoracle.delta_t(cfg, U))
+        secs, kind, threads = time.perf_counter() - t0, "port", pyoracle.threads()
+    value = n * n * steps / secs / 1e6
+    return {"value": round(value, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{steps} RK3 steps of the same Noh workload at {n}x{n} "
+                      f"(dt + 3 stages, IO excluded), {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4096, help="grid cells per side")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has "
+              f"no multi-GPU path; see DESIGN.md for how the step would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from overheatingproblem_amd import cases
+    n = args.n
+    cfg, U0 = cases.noh(n, positivity=True)
+    cells = n * n
+    log(f"bench: noh {n}x{n} fp64 warmup={args.warmup} steps={args.steps}")
+    r = run_gpu(cfg, U0, args.steps, args.warmup)
+    ms_per_step = r["gpu_ms"] / args.steps
+    value = cells * args.steps / (r["gpu_ms"] / 1e3) / 1e6
+    log(f"bench: {ms_per_step:.3f} ms/step, {value:.1f} Mcell-updates/s, faults={r['faults']}")
+
+    kt = r["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms, dom_n = kt[dom]
+    avg_ms = dom_ms / max(1, dom_n)
+    faces = (n + 1) * n
+    if dom in ("flux_x", "flux_y"):
+        alg_bytes = FLUX_BYTES_PER_FACE * faces
+    else:
+        alg_bytes = 32.0 * 2 * cells
+    achieved = alg_bytes / (avg_ms / 1e3) / 1e9
+    step_gbs = STEP_BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
+
+    rho = r["U"][0, 4:-4, 4:-4]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"noh-{n}x{n}", "scheme": "WENO5-char(Z) + HLLC, SSP-RK3",
+                   "cfl": 0.4, "gamma": "5/3", "p0": 1e-6, "form": "fp64",
+                   "positivity_fallback": True, "cells": cells},
+        "roofline": {"bound": "hbm", "kernel": f"k_flux<{dom}>" if dom.startswith("flux") else dom,
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "launches": dom_n,
+                     "alg_bytes_per_launch": alg_bytes},
+        "step_hbm": {"achieved_gbs": round(step_gbs, 2), "frac": round(step_gbs / HBM_PEAK_GBS, 5),
+                     "bytes_per_cell_update": STEP_BYTES_PER_CELL},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "wall_s": round(r["wall_s"], 4),
+        "faults": r["faults"],
+        "finite": bool(np.isfinite(rho).all()),
+    }
+    if not args.no_cpu:
+        log(f"bench: cpu baseline ({args.cpu_steps} steps at {n}^2) ...")
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps, host_cores())
+        except Exception as e:  # the GPU figure stands without it
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

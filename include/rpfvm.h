@@ -1,0 +1,180 @@
+/*
+ * rpfvm.h -- C ABI of the MI355X (gfx950) finite-volume step for the 2-D
+ * compressible Euler equations.  Drop-in for the FP64 CPU/OpenMP step of
+ * RP_FVM (xujf520/OverheatingProblem).
+ *
+ * Everything here is plain C: pointers, sizes and ints, no C++ or torch types.
+ * A caller owns every host buffer; device buffers belong to the context.
+ *
+ * Layout contract (the reference's, src/RP_FVM.c:124-142):
+ *   a host state is double U[4][R][C], R = nx + 8 (x, first index),
+ *   C = ny + 8 (y, contiguous index), 4 ghost layers per side, variables
+ *   (rho, rho*u, rho*v, E).
+ *
+ * Return codes: 0 = ok; < 0 = invalid argument or HIP runtime error;
+ * > 0 = numerical fault detected on the device (see RPFVM_FAULT_*).
+ * The library never calls exit().
+ *
+ * One host thread per context.  All work of a context is ordered on the
+ * context's own HIP stream.
+ */
+#ifndef RPFVM_H
+#define RPFVM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPFVM_GHOST 4      /* GhostCell, include/Golbal.h:49 */
+#define RPFVM_NVAR 4       /* var, include/Golbal.h:33 */
+
+/* BC_Type, include/Boundary_Condition.h:22-28 (same numbering) */
+enum {
+    RPFVM_BC_OUTFLOW = 0,
+    RPFVM_BC_REFLECTION = 1,
+    RPFVM_BC_PERIODICITY = 2,
+    RPFVM_BC_INFLOW = 3,
+    RPFVM_BC_FIXED_VALUE = 4
+};
+
+/* Riemann solver ids as the "scheme" control key, include/CFD_convection.h:89-117.
+ * Any id that is not 1, 2 or 3 selects LLF, as the reference's default branch. */
+enum { RPFVM_SCHEME_HLL = 1, RPFVM_SCHEME_HLLC = 2, RPFVM_SCHEME_ROE = 3, RPFVM_SCHEME_LLF = 10 };
+
+/* Arithmetic form of the step. */
+enum {
+    RPFVM_PRECISION_FP64 = 0,  /* bit-compatible with the reference FP64 path */
+    RPFVM_PRECISION_MIXED = 1  /* FP32 reconstruction/projection/flux, FP64 state and dt */
+};
+
+/* Fault bits reported by rpfvm_get_faults (also returned by rpfvm_sync). */
+enum {
+    RPFVM_FAULT_HLL_BAD_STATE = 1,   /* HLL face with rho <= 0, NaN rho or NaN wave speed
+                                        (the reference prints and skips: scheme.h:427-436,500-503) */
+    RPFVM_FAULT_NONFINITE_DT = 2,    /* dt reduction produced a non-finite or non-positive dt */
+    RPFVM_FAULT_MIXED_FALLBACK = 4   /* mixed form recomputed at least one face in FP64
+                                        (informational, not an error) */
+};
+
+/* Error codes (< 0). */
+enum {
+    RPFVM_OK = 0,
+    RPFVM_ERR_INVALID = -1,      /* bad argument or configuration */
+    RPFVM_ERR_UNSUPPORTED = -2,  /* configuration not implemented on the device */
+    RPFVM_ERR_HIP = -3,          /* HIP runtime error (see rpfvm_last_error) */
+    RPFVM_ERR_NOMEM = -4
+};
+
+/*
+ * Step configuration.  The ControlFile.txt keys (src/RP_FVM.c:434-529) map as
+ *   L_nx -> nx, L_ny -> ny, Time_ADM -> time_order, scheme -> scheme,
+ *   Recon_Accur -> recon_order, Characteriz -> characteristic, Source -> source,
+ *   M_gamma -> gamma, CFL -> cfl, Gravity -> gravity,
+ * and the process globals bc_config / inflow_state / *_fixed_value_state
+ * (src/RP_FVM.c:86, include/Boundary_Condition.h:49, include/initialize.h:1298-1301)
+ * become bc[4] / inflow_state / fixed_state.
+ */
+typedef struct rpfvm_config {
+    int32_t nx, ny;            /* interior cells in x (rows) and y (columns) */
+    int32_t time_order;        /* 1, 2 or 3 (RK1_TimeAd, RK2_TimeAd, RK3_TimeAd) */
+    int32_t scheme;            /* RPFVM_SCHEME_* */
+    int32_t recon_order;       /* 1 Godunov, 2 TVD, 3 WENO3, 5 WENO5 */
+    int32_t characteristic;    /* 0 component-wise, 1 characteristic-variable reconstruction */
+    int32_t source;            /* gravity source term on/off */
+    int32_t precision;         /* RPFVM_PRECISION_* */
+    int32_t positivity;        /* 0: reference behaviour.  1: a face whose reconstructed
+                                  rho or p is not positive (or not finite) falls back to
+                                  first-order (piecewise-constant) states.  Not in the
+                                  reference, which has no positivity safeguard. */
+    int32_t reserved0;         /* must be 0 */
+    int32_t bc[4];             /* left, right, bottom, top: RPFVM_BC_* */
+    double dx, dy;
+    double gamma, cfl, gravity;
+    double inflow_state[4];    /* conserved (rho, rho u, rho v, E) */
+    double fixed_state[4][4];  /* [side][var], side order left, right, bottom, top */
+} rpfvm_config;
+
+typedef struct rpfvm_ctx rpfvm_ctx;
+
+/* Fill a config with the reference's effective defaults: RK3, HLLC, WENO5
+ * component-wise (Recon_Accur = 5, Characteriz = false, include/Golbal.h:58,60),
+ * gamma 1.4, CFL 0.5, outflow on all sides, inflow/fixed states as the
+ * reference's initialisers (include/Boundary_Condition.h:49). */
+void rpfvm_config_default(rpfvm_config *cfg);
+
+/* Context lifecycle.  Allocates device state and scratch on the current HIP
+ * device.  Replaces the per-call malloc/free of RK*_TimeAd and
+ * Flux_Reconstruction_RP (include/time_advance.h:791-821, CFD_convection.h:43-45). */
+int rpfvm_create(const rpfvm_config *cfg, rpfvm_ctx **out);
+int rpfvm_destroy(rpfvm_ctx *ctx);
+
+/* Host <-> device state transfer, reference layout double[4][R][C]. */
+int rpfvm_set_state(rpfvm_ctx *ctx, const double *U);
+int rpfvm_get_state(rpfvm_ctx *ctx, double *U);
+
+/* Get_Delta_T_2D, include/function.h:144-175: dt = CFL*min(dx,dy)/(max(|u|+a)+max(|v|+a))
+ * over the interior.  Synchronises the stream and returns dt on the host. */
+int rpfvm_get_delta_t(rpfvm_ctx *ctx, double *dt);
+
+/* RK1_TimeAd / RK2_TimeAd / RK3_TimeAd (include/time_advance.h:643-882), chosen by
+ * cfg.time_order: ghost fill of every stage input (including the state itself),
+ * spatial operator, stage combine.  Asynchronous on the context stream. */
+int rpfvm_time_advance(rpfvm_ctx *ctx, double dt);
+
+/* Boundary_Conditions, include/Boundary_Condition.h:963-980: ghost fill of the
+ * state in place, sides in the order left, right, bottom, top.  Asynchronous. */
+int rpfvm_boundary_conditions(rpfvm_ctx *ctx);
+
+/* Source_Gravity + Boundary_Conditions + Flux_Reconstruction_RP + Space_Discrete_Item
+ * (include/time_advance.h:824-829) on the current state: ghost fill in place, then
+ * L = -(F_i - F_{i-1})/dx - (G_j - G_{j-1})/dy + S.  Writes host arrays
+ * L[4][R][C] (interior), F[4][R][C] (x faces i+1/2 at [i], i = 3..R-5) and
+ * G[4][R][C] (y faces j+1/2 at [j]); any of them may be NULL.  Synchronous. */
+int rpfvm_space_operator(rpfvm_ctx *ctx, double *L, double *F, double *G);
+
+/* Time-controlled loop of the reference driver (src/RP_FVM.c:179-191), on the
+ * device: repeat { dt = Get_Delta_T_2D; clip to t_stop; step } until *t reaches
+ * t_stop or max_steps steps have run.  dt and t never leave the device inside the
+ * loop; *t and *steps are updated on return.  Synchronous. */
+int rpfvm_advance_to(rpfvm_ctx *ctx, double t_stop, int max_steps, double *t, int *steps);
+
+/* Wait for the context stream and return the fault bits accumulated since the last
+ * call (0 when clean); the bits are cleared. */
+int rpfvm_sync(rpfvm_ctx *ctx);
+int rpfvm_get_faults(rpfvm_ctx *ctx, int *faults);
+
+/* Device access for callers that keep data on the GPU.  The state lives in four
+ * planes of R rows with a row pitch of *ld doubles (ld >= C); plane v starts at
+ * base + v * plane_stride.  The stream is a hipStream_t. */
+int rpfvm_device_state(rpfvm_ctx *ctx, double **base, size_t *ld, size_t *plane_stride);
+int rpfvm_stream(rpfvm_ctx *ctx, void **stream);
+
+/* Kernel timing (no counterpart in the reference, which only times whole steps with
+ * omp_get_wtime, src/RP_FVM.c:181-216).  While enabled, every kernel launch of the
+ * context is bracketed by HIP events on the context stream.  rpfvm_kernel_times
+ * synchronises, returns total milliseconds and launch counts per kernel class
+ * (RPFVM_KCLASS_*), and resets them. */
+enum {
+    RPFVM_KCLASS_BC = 0,       /* ghost fill */
+    RPFVM_KCLASS_FLUX_X = 1,   /* x-face reconstruction + Riemann flux */
+    RPFVM_KCLASS_FLUX_Y = 2,   /* y-face reconstruction + Riemann flux */
+    RPFVM_KCLASS_UPDATE = 3,   /* residual + RK stage combine */
+    RPFVM_KCLASS_DT = 4,       /* CFL reduction */
+    RPFVM_KCLASS_COUNT = 5
+};
+int rpfvm_set_kernel_timing(rpfvm_ctx *ctx, int enabled);
+int rpfvm_kernel_times(rpfvm_ctx *ctx, double ms[RPFVM_KCLASS_COUNT], int counts[RPFVM_KCLASS_COUNT]);
+
+/* Message of the last error seen on this thread. */
+const char *rpfvm_last_error(void);
+
+/* Library version string. */
+const char *rpfvm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPFVM_H */

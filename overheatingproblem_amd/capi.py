@@ -1,0 +1,193 @@
+"""ctypes binding of the gfx950 library (overheatingproblem_amd/lib/librpfvm.so).
+
+``Solver`` mirrors the reference's step interface (the functions its driver calls,
+src/RP_FVM.c:179-209):
+
+    Get_Delta_T_2D           (include/function.h:144)        -> Solver.get_delta_t()
+    RK1/RK2/RK3_TimeAd       (include/time_advance.h:643-882) -> Solver.time_advance(dt)
+    Boundary_Conditions      (include/Boundary_Condition.h:963) -> Solver.boundary_conditions()
+    Flux_Reconstruction_RP + Space_Discrete_Item
+                             (include/CFD_convection.h:38, include/scheme.h:3758)
+                                                               -> Solver.space_operator()
+
+There is no CPU fallback: without the built library (or without a GPU) every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import Config
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librpfvm.so")
+
+# Every entry point include/rpfvm.h declares.
+EXPORTS = (
+    "rpfvm_config_default", "rpfvm_create", "rpfvm_destroy", "rpfvm_set_state",
+    "rpfvm_get_state", "rpfvm_get_delta_t", "rpfvm_time_advance", "rpfvm_boundary_conditions",
+    "rpfvm_space_operator", "rpfvm_advance_to", "rpfvm_sync", "rpfvm_get_faults",
+    "rpfvm_device_state", "rpfvm_stream", "rpfvm_set_kernel_timing", "rpfvm_kernel_times",
+    "rpfvm_last_error", "rpfvm_version",
+)
+
+KERNEL_CLASSES = ("bc", "flux_x", "flux_y", "update", "dt")
+
+
+class RpfvmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load librpfvm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RpfvmError(f"{LIB_PATH} is missing: build it with python -m overheatingproblem_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    szp = ctypes.POINTER(ctypes.c_size_t)
+    L.rpfvm_config_default.argtypes = [ctypes.POINTER(Config)]
+    L.rpfvm_config_default.restype = None
+    L.rpfvm_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.rpfvm_destroy.argtypes = [vp]
+    L.rpfvm_set_state.argtypes = [vp, dp]
+    L.rpfvm_get_state.argtypes = [vp, dp]
+    L.rpfvm_get_delta_t.argtypes = [vp, dp]
+    L.rpfvm_time_advance.argtypes = [vp, ctypes.c_double]
+    L.rpfvm_boundary_conditions.argtypes = [vp]
+    L.rpfvm_space_operator.argtypes = [vp, dp, dp, dp]
+    L.rpfvm_advance_to.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp, ip]
+    L.rpfvm_sync.argtypes = [vp]
+    L.rpfvm_get_faults.argtypes = [vp, ip]
+    L.rpfvm_device_state.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), szp, szp]
+    L.rpfvm_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.rpfvm_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.rpfvm_kernel_times.argtypes = [vp, dp, ip]
+    L.rpfvm_last_error.restype = ctypes.c_char_p
+    L.rpfvm_version.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("rpfvm_config_default", "rpfvm_last_error", "rpfvm_version"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        msg = lib().rpfvm_last_error().decode(errors="replace")
+        raise RpfvmError(f"rpfvm error {rc}: {msg}")
+    return rc
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    """One device context of the step (``rpfvm_ctx``)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg.copy()
+        self._ctx = ctypes.c_void_p()
+        _check(lib().rpfvm_create(ctypes.byref(self.cfg), ctypes.byref(self._ctx)))
+
+    # context management ------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            lib().rpfvm_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return self.cfg.shape
+
+    # state -------------------------------------------------------------------
+    def set_state(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.shape:
+            raise ValueError(f"state shape {U.shape} != {self.shape}")
+        _check(lib().rpfvm_set_state(self._ctx, _dptr(U)))
+
+    def get_state(self) -> np.ndarray:
+        U = np.empty(self.shape, dtype=np.float64)
+        _check(lib().rpfvm_get_state(self._ctx, _dptr(U)))
+        return U
+
+    # the reference's step interface -----------------------------------------
+    def get_delta_t(self) -> float:
+        dt = ctypes.c_double()
+        _check(lib().rpfvm_get_delta_t(self._ctx, ctypes.byref(dt)))
+        return dt.value
+
+    def time_advance(self, dt: float) -> None:
+        _check(lib().rpfvm_time_advance(self._ctx, ctypes.c_double(dt)))
+
+    def boundary_conditions(self) -> None:
+        _check(lib().rpfvm_boundary_conditions(self._ctx))
+
+    def space_operator(self):
+        """Returns (L, F, G) as host arrays in the reference layout."""
+        L = np.zeros(self.shape)
+        F = np.zeros(self.shape)
+        G = np.zeros(self.shape)
+        _check(lib().rpfvm_space_operator(self._ctx, _dptr(L), _dptr(F), _dptr(G)))
+        return L, F, G
+
+    def advance_to(self, t_stop: float, max_steps: int, t: float = 0.0):
+        tt = ctypes.c_double(t)
+        n = ctypes.c_int(0)
+        _check(lib().rpfvm_advance_to(self._ctx, ctypes.c_double(t_stop), int(max_steps),
+                                      ctypes.byref(tt), ctypes.byref(n)))
+        return tt.value, n.value
+
+    def sync(self) -> int:
+        """Wait for the stream; returns (and clears) the fault bits."""
+        return _check(lib().rpfvm_sync(self._ctx))
+
+    def device_state(self):
+        base = ctypes.c_void_p()
+        ld = ctypes.c_size_t()
+        plane = ctypes.c_size_t()
+        _check(lib().rpfvm_device_state(self._ctx, ctypes.byref(base), ctypes.byref(ld),
+                                        ctypes.byref(plane)))
+        return base.value, ld.value, plane.value
+
+    def set_kernel_timing(self, enabled: bool) -> None:
+        _check(lib().rpfvm_set_kernel_timing(self._ctx, int(bool(enabled))))
+
+    def kernel_times(self) -> dict:
+        """{class: (total_ms, launches)} since the last call; synchronises."""
+        ms = (ctypes.c_double * len(KERNEL_CLASSES))()
+        n = (ctypes.c_int * len(KERNEL_CLASSES))()
+        _check(lib().rpfvm_kernel_times(self._ctx, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().rpfvm_stream(self._ctx, ctypes.byref(s)))
+        return s.value or 0
+
+
+def version() -> str:
+    return lib().rpfvm_version().decode()

@@ -1,0 +1,425 @@
+// fvm_api.cpp -- the C ABI of include/rpfvm.h on top of the gfx950 kernels.
+//
+// A context owns its device buffers (state, two stage arrays, two face-flux arrays,
+// a residual array and reduction scratch), allocated once at rpfvm_create: the
+// reference instead mallocs and zeroes six state-sized arrays per RK call
+// (include/time_advance.h:791-821) and three per spatial operator
+// (include/CFD_convection.h:43-45).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/rpfvm.h"
+#include "fvm_kernels.hpp"
+
+using namespace fvm;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(RPFVM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kFrontPad = 12;  // makes column GC of every row 128-byte aligned
+
+}  // namespace
+
+struct rpfvm_ctx {
+    rpfvm_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geom g{};
+    Phys ph{};
+    FluxPair flux{};
+    BcSide bc[4]{};
+    void* pool = nullptr;
+    double *U = nullptr, *U1 = nullptr, *U2 = nullptr, *F = nullptr, *G = nullptr, *L = nullptr;
+    double* partial = nullptr;
+    double* scal = nullptr;        // [0] dt, [1] t, [2] t_stop
+    unsigned* faults = nullptr;
+    double* host_scal = nullptr;   // pinned mirror of scal
+    unsigned* host_faults = nullptr;
+    // kernel timing (rpfvm_set_kernel_timing)
+    bool timing = false;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> spare;
+    double ms[RPFVM_KCLASS_COUNT] = {};
+    int counts[RPFVM_KCLASS_COUNT] = {};
+};
+
+namespace {
+
+hipEvent_t take_event(rpfvm_ctx* x) {
+    if (!x->spare.empty()) {
+        hipEvent_t e = x->spare.back();
+        x->spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the launches issued during its lifetime with a pair of events.
+struct Timed {
+    rpfvm_ctx* x;
+    int cls;
+    hipEvent_t a = nullptr;
+    Timed(rpfvm_ctx* ctx, int c) : x(ctx), cls(c) {
+        if (x->timing && (a = take_event(x))) (void)hipEventRecord(a, x->stream);
+    }
+    ~Timed() {
+        if (!a) return;
+        hipEvent_t b = take_event(x);
+        if (!b) return;
+        (void)hipEventRecord(b, x->stream);
+        x->pending.push_back({cls, a, b});
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* rpfvm_last_error(void) { return g_err; }
+
+const char* rpfvm_version(void) { return "rpfvm-gfx950 0.1.0"; }
+
+void rpfvm_config_default(rpfvm_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->nx = c->ny = 100;
+    c->time_order = 3;
+    c->scheme = RPFVM_SCHEME_HLLC;
+    c->recon_order = 5;
+    c->characteristic = 0;
+    c->dx = c->dy = 0.01;
+    c->gamma = 1.4;
+    c->cfl = 0.5;
+    c->gravity = 1.0;
+    // include/Boundary_Condition.h:49 and include/initialize.h:1298-1301
+    const double mach3[4] = {1.4, 1.4 * 3.0, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 3.0 * 3.0};
+    for (int k = 0; k < 4; k++) {
+        c->inflow_state[k] = mach3[k];
+        for (int s = 0; s < 4; s++) c->fixed_state[s][k] = mach3[k];
+    }
+}
+
+static int validate(const rpfvm_config* c) {
+    if (!c) return fail(RPFVM_ERR_INVALID, "null config");
+    if (c->nx < 1 || c->ny < 1) return fail(RPFVM_ERR_INVALID, "nx, ny must be >= 1");
+    if ((long)(c->nx + 2 * GC) * (c->ny + 2 * GC) > (1L << 31))
+        return fail(RPFVM_ERR_INVALID, "grid too large");
+    if (c->time_order < 1 || c->time_order > 3)
+        return fail(RPFVM_ERR_INVALID, "time_order must be 1, 2 or 3");
+    if (c->recon_order != 1 && c->recon_order != 2 && c->recon_order != 3 && c->recon_order != 5)
+        return fail(RPFVM_ERR_INVALID, "recon_order must be 1, 2, 3 or 5");
+    if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
+        return fail(RPFVM_ERR_UNSUPPORTED, "scheme %d is not implemented on the device", c->scheme);
+    if (c->precision != RPFVM_PRECISION_FP64)
+        return fail(RPFVM_ERR_UNSUPPORTED, "precision %d is not implemented", c->precision);
+    for (int s = 0; s < 4; s++)
+        if (c->bc[s] < 0 || c->bc[s] > 4) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
+    if (!(c->dx > 0) || !(c->dy > 0) || !(c->gamma > 1.0))
+        return fail(RPFVM_ERR_INVALID, "dx, dy must be > 0 and gamma > 1");
+    return 0;
+}
+
+int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
+    if (!out) return fail(RPFVM_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int e = validate(cfg);
+    if (e) return e;
+    rpfvm_ctx* x = new (std::nothrow) rpfvm_ctx();
+    if (!x) return fail(RPFVM_ERR_NOMEM, "out of host memory");
+    x->cfg = *cfg;
+    const rpfvm_config& c = x->cfg;
+    x->g.R = c.nx + 2 * GC;
+    x->g.C = c.ny + 2 * GC;
+    x->g.ld = ((x->g.C + 15) / 16) * 16;
+    x->g.plane = (size_t)x->g.R * x->g.ld;
+    x->ph.gamma = c.gamma;
+    x->ph.dx = c.dx;
+    x->ph.dy = c.dy;
+    x->ph.cfl = c.cfl;
+    x->ph.gravity = c.gravity;
+    x->ph.source = c.source ? 1 : 0;
+    x->ph.positivity = c.positivity ? 1 : 0;
+    x->flux = select_flux(c.recon_order, c.characteristic != 0, c.scheme);
+    for (int s = 0; s < 4; s++) {
+        BcSide& b = x->bc[s];
+        b.side = s;
+        b.type = c.bc[s];
+        b.lo = GC;
+        const bool lr = s < 2;
+        const int last = lr ? x->g.C - GC : x->g.R - GC;  // outflow loops are inclusive
+        b.hi = b.type == BC_OUT ? last : last - 1;
+        for (int k = 0; k < 4; k++)
+            b.value[k] = b.type == BC_INFLOW ? c.inflow_state[k] : c.fixed_state[s][k];
+    }
+
+    auto cleanup = [&](int code) {
+        rpfvm_destroy(x);
+        return code;
+    };
+    hipError_t he = hipGetDevice(&x->device);
+    if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
+    he = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)));
+
+    const size_t arr = 4 * x->g.plane + 32;             // doubles per state-sized array
+    const size_t arr_bytes = ((arr * sizeof(double) + 255) / 256) * 256;
+    const int nb = dt_blocks(x->g);
+    const size_t small = 256 * 4 + (size_t)nb * 2 * sizeof(double) + 256;
+    he = hipMalloc(&x->pool, 6 * arr_bytes + small);
+    if (he != hipSuccess)
+        return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", 6 * arr_bytes + small, hipGetErrorString(he)));
+    char* p = static_cast<char*>(x->pool);
+    double** arrays[6] = {&x->U, &x->U1, &x->U2, &x->F, &x->G, &x->L};
+    for (int k = 0; k < 6; k++) *arrays[k] = reinterpret_cast<double*>(p + k * arr_bytes) + kFrontPad;
+    char* q = p + 6 * arr_bytes;
+    x->scal = reinterpret_cast<double*>(q);
+    x->faults = reinterpret_cast<unsigned*>(q + 256);
+    x->partial = reinterpret_cast<double*>(q + 512);
+    he = hipMemsetAsync(x->pool, 0, 6 * arr_bytes + small, x->stream);
+    if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
+    he = hipHostMalloc(reinterpret_cast<void**>(&x->host_scal), 256, hipHostMallocDefault);
+    if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(he)));
+    x->host_faults = reinterpret_cast<unsigned*>(x->host_scal + 8);
+    he = hipStreamSynchronize(x->stream);
+    if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "sync: %s", hipGetErrorString(he)));
+    *out = x;
+    return 0;
+}
+
+int rpfvm_destroy(rpfvm_ctx* x) {
+    if (!x) return 0;
+    if (x->stream) (void)hipStreamSynchronize(x->stream);
+    for (auto& p : x->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : x->spare) (void)hipEventDestroy(e);
+    if (x->pool) (void)hipFree(x->pool);
+    if (x->host_scal) (void)hipHostFree(x->host_scal);
+    if (x->stream) (void)hipStreamDestroy(x->stream);
+    delete x;
+    return 0;
+}
+
+static int copy_in(rpfvm_ctx* x, double* dev, const double* host) {
+    HIPCHK(hipMemcpy2DAsync(dev, x->g.ld * sizeof(double), host, x->g.C * sizeof(double),
+                            x->g.C * sizeof(double), 4 * (size_t)x->g.R, hipMemcpyHostToDevice,
+                            x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+static int copy_out(rpfvm_ctx* x, double* host, const double* dev) {
+    HIPCHK(hipMemcpy2DAsync(host, x->g.C * sizeof(double), dev, x->g.ld * sizeof(double),
+                            x->g.C * sizeof(double), 4 * (size_t)x->g.R, hipMemcpyDeviceToHost,
+                            x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int rpfvm_set_state(rpfvm_ctx* x, const double* U) {
+    if (!x || !U) return fail(RPFVM_ERR_INVALID, "null argument");
+    return copy_in(x, x->U, U);
+}
+
+int rpfvm_get_state(rpfvm_ctx* x, double* U) {
+    if (!x || !U) return fail(RPFVM_ERR_INVALID, "null argument");
+    return copy_out(x, U, x->U);
+}
+
+static void enqueue_bc(rpfvm_ctx* x, double* Y) {
+    Timed t(x, RPFVM_KCLASS_BC);
+    for (int s = 0; s < 4; s++) launch_bc_side(Y, x->g, x->bc[s], x->stream);
+}
+
+static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
+    {
+        Timed t(x, RPFVM_KCLASS_FLUX_X);
+        x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->stream);
+    }
+    {
+        Timed t(x, RPFVM_KCLASS_FLUX_Y);
+        x->flux.y(Y, x->G, x->g, x->ph, x->faults, x->stream);
+    }
+}
+
+static void enqueue_update(rpfvm_ctx* x, const double* Un, const double* Y, double* out,
+                           double dt, int kind) {
+    Timed t(x, RPFVM_KCLASS_UPDATE);
+    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->stream);
+}
+
+static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
+    Timed t(x, RPFVM_KCLASS_DT);
+    launch_dt(x->U, x->g, x->ph, x->partial, x->scal, clock, x->faults, x->stream);
+}
+
+// One stage: ghost fill of Y, face fluxes, residual and stage combine into out.
+static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
+                          int kind) {
+    enqueue_bc(x, Y);
+    enqueue_fluxes(x, Y);
+    enqueue_update(x, Un, Y, out, dt, kind);
+}
+
+static int enqueue_step(rpfvm_ctx* x, double dt) {
+    switch (x->cfg.time_order) {
+        case 1:
+            enqueue_stage(x, x->U, x->U, x->U, dt, UPD_EULER);
+            break;
+        case 2:
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
+            enqueue_stage(x, x->U, x->U1, x->U, dt, UPD_RK2_2);
+            break;
+        default:
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
+            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2);
+            enqueue_stage(x, x->U, x->U2, x->U, dt, UPD_RK3_3);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int rpfvm_time_advance(rpfvm_ctx* x, double dt) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    return enqueue_step(x, dt);
+}
+
+int rpfvm_boundary_conditions(rpfvm_ctx* x) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    enqueue_bc(x, x->U);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int rpfvm_get_delta_t(rpfvm_ctx* x, double* dt) {
+    if (!x || !dt) return fail(RPFVM_ERR_INVALID, "null argument");
+    enqueue_dt(x, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x->host_scal, x->scal, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    *dt = x->host_scal[0];
+    return 0;
+}
+
+int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    enqueue_bc(x, x->U);
+    enqueue_fluxes(x, x->U);
+    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL);
+    HIPCHK(hipGetLastError());
+    int e = 0;
+    if (L && (e = copy_out(x, L, x->L))) return e;
+    if (F && (e = copy_out(x, F, x->F))) return e;
+    if (G && (e = copy_out(x, G, x->G))) return e;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int rpfvm_advance_to(rpfvm_ctx* x, double t_stop, int max_steps, double* t, int* steps) {
+    if (!x || !t) return fail(RPFVM_ERR_INVALID, "null argument");
+    double time = *t;
+    int n = 0;
+    // src/RP_FVM.c:179-191 with next_output_time == Tmax == t_stop
+    while (time < t_stop && n < max_steps) {
+        double dt;
+        int e = rpfvm_get_delta_t(x, &dt);
+        if (e) return e;
+        if (time + dt > t_stop) dt = t_stop - time;
+        if (time + dt >= t_stop) dt = t_stop - time;
+        if ((e = enqueue_step(x, dt))) return e;
+        time = time + dt;
+        n++;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    *t = time;
+    if (steps) *steps = n;
+    return 0;
+}
+
+int rpfvm_get_faults(rpfvm_ctx* x, int* faults) {
+    if (!x || !faults) return fail(RPFVM_ERR_INVALID, "null argument");
+    HIPCHK(hipMemcpyAsync(x->host_faults, x->faults, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemsetAsync(x->faults, 0, sizeof(unsigned), x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    *faults = (int)*x->host_faults;
+    return 0;
+}
+
+int rpfvm_sync(rpfvm_ctx* x) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    int f = 0;
+    int e = rpfvm_get_faults(x, &f);
+    return e ? e : f;
+}
+
+int rpfvm_set_kernel_timing(rpfvm_ctx* x, int enabled) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    x->timing = enabled != 0;
+    return 0;
+}
+
+int rpfvm_kernel_times(rpfvm_ctx* x, double* ms, int* counts) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (auto& p : x->pending) {
+        float f = 0.0f;
+        HIPCHK(hipEventElapsedTime(&f, p.a, p.b));
+        x->ms[p.cls] += f;
+        x->counts[p.cls] += 1;
+        x->spare.push_back(p.a);
+        x->spare.push_back(p.b);
+    }
+    x->pending.clear();
+    for (int k = 0; k < RPFVM_KCLASS_COUNT; k++) {
+        if (ms) ms[k] = x->ms[k];
+        if (counts) counts[k] = x->counts[k];
+        x->ms[k] = 0.0;
+        x->counts[k] = 0;
+    }
+    return 0;
+}
+
+int rpfvm_device_state(rpfvm_ctx* x, double** base, size_t* ld, size_t* plane_stride) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    if (base) *base = x->U;
+    if (ld) *ld = (size_t)x->g.ld;
+    if (plane_stride) *plane_stride = x->g.plane;
+    return 0;
+}
+
+int rpfvm_stream(rpfvm_ctx* x, void** stream) {
+    if (!x || !stream) return fail(RPFVM_ERR_INVALID, "null argument");
+    *stream = (void*)x->stream;
+    return 0;
+}
+
+}  // extern "C"

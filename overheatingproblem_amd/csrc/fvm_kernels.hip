@@ -1,0 +1,280 @@
+// fvm_kernels.hip -- gfx950 kernels of the FP64 finite-volume step.
+//
+// Per RK stage (include/time_advance.h:823-872):
+//   k_bc_side  x4   ghost fill of the stage input, sides L, R, B, T in order
+//   k_flux<1>       x-face fluxes  F[i+1/2][j]: 6-cell stencil -> reconstruction -> Riemann
+//   k_flux<2>       y-face fluxes  G[i][j+1/2]
+//   k_update        L = -(dF)/dx - (dG)/dy + S and the SSP-RK stage combine
+// and per step the CFL reduction k_dt_partial + k_dt_final (include/function.h:144-175).
+//
+// Layout in HBM: four planes (rho, rho u, rho v, E) of R rows by `ld` doubles, ld a
+// multiple of 16 and the first interior column 128-byte aligned; a wavefront's 64
+// lanes walk the contiguous (y) index, so every load and store is coalesced.
+#include <hip/hip_runtime.h>
+
+#include "fvm_kernels.hpp"
+#include "fvm_math.hpp"
+
+namespace fvm {
+
+// --------------------------------------------------------------- ghost fill
+// One side of Boundary_Conditions (include/Boundary_Condition.h:963-980).  A thread
+// owns one (variable, tangential index) pair and writes the four ghost layers in the
+// reference's order, so a side that reads its own ghosts (periodic or reflecting
+// with fewer than four interior cells) sees the same values as the reference loop.
+__global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s) {
+    const int t = s.lo + blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (t > s.hi) return;
+    double* P = U + (size_t)v * g.plane;
+    const bool lr = s.side < 2;
+    const double sign = (s.type == BC_REFLECT && v == (lr ? 1 : 2)) ? -1.0 : 1.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        int gi, si = -1;
+        switch (s.side) {
+            case 0: gi = GC - 1 - m; break;
+            case 1: gi = g.R - GC + m; break;
+            case 2: gi = GC - 1 - m; break;
+            default: gi = g.C - GC + m; break;
+        }
+        double val = s.value[v];
+        switch (s.type) {
+            case BC_OUT:
+                si = s.side == 0 ? GC : s.side == 1 ? g.R - GC - 1 : s.side == 2 ? GC : g.C - GC - 1;
+                break;
+            case BC_PERIODIC:
+                si = s.side == 0 ? g.R - GC - 1 - m : s.side == 1 ? GC + m
+                   : s.side == 2 ? g.C - GC - 1 - m : GC + m;
+                break;
+            case BC_REFLECT:
+                si = s.side == 0 ? GC + m : s.side == 1 ? g.R - GC - 1
+                   : s.side == 2 ? GC : g.C - GC - 1 - m;
+                break;
+            default:
+                break;  // inflow / fixed value: s.value
+        }
+        if (si >= 0) {
+            const double x = lr ? P[(size_t)si * g.ld + t] : P[(size_t)t * g.ld + si];
+            val = sign < 0.0 ? -x : x;
+        }
+        if (lr) P[(size_t)gi * g.ld + t] = val;
+        else P[(size_t)t * g.ld + gi] = val;
+    }
+}
+
+// ------------------------------------------------------------ face fluxes
+// One thread per face.  DIR 1: faces (i+1/2, j), i = GC-1..R-GC-1, j = GC..C-GC-1;
+// DIR 2: faces (i, j+1/2), i = GC..R-GC-1, j = GC-1..C-GC-1.  The flux is stored at
+// the face's left cell, [i][j], as the reference's Flux array.
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux(const double* __restrict__ Y, double* __restrict__ F,
+                                              Geom g, Phys ph, unsigned* __restrict__ faults) {
+    const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
+    const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.R - GC || j >= g.C - GC) return;
+    const size_t c0 = (size_t)i * g.ld + j;
+    const size_t step = DIR == 1 ? (size_t)g.ld : 1;
+    double st[6][4];
+#pragma unroll
+    for (int n = 0; n < 6; n++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2) * step];
+    double UL[4], UR[4];
+    reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.positivity && !(admissible(ph.gamma, UL) && admissible(ph.gamma, UR))) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            UL[v] = st[2][v];
+            UR[v] = st[3][v];
+        }
+    }
+    double f[4];
+    if (riemann<DIR, SOLVER>(ph.gamma, UL, UR, f)) atomicOr(faults, 1u);
+#pragma unroll
+    for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+}
+
+// ------------------------------------------------------------------ update
+// Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
+// fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872).
+// `out` may alias `Un` (last stage writes the state in place): each thread reads
+// its own cell before writing it.
+__global__ __launch_bounds__(256) void k_update(const double* Un, const double* Y, double* out, const double* __restrict__ F,
+                                                const double* __restrict__ G, Geom g, Phys ph,
+                                                double dt, int kind) {
+    const int j = GC + blockIdx.x * 64 + threadIdx.x;
+    const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.R - GC || j >= g.C - GC) return;
+    const size_t c = (size_t)i * g.ld + j;
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ph.source) {
+        const double rho = Y[c];
+        const double v = Y[2 * g.plane + c] / rho;
+        S[2] = rho * ph.gravity;
+        S[3] = rho * ph.gravity * v;
+    }
+    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const size_t o = (size_t)k * g.plane + c;
+        const double L = -(F[o] - F[o - g.ld]) * idx - (G[o] - G[o - 1]) * idy + S[k];
+        double r;
+        switch (kind) {
+            case UPD_RESIDUAL: r = L; break;                                   // L only
+            case UPD_EULER: r = Y[o] + dt * L; break;                          // U1 = Y + dt L
+            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (Y[o] + dt * L); break;
+            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (Y[o] + dt * L); break;
+            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (Y[o] + dt * L); break;  // UPD_RK3_3
+        }
+        out[o] = r;
+    }
+}
+
+// ------------------------------------------------------------------- CFL dt
+// Get_Delta_T_2D (include/function.h:144-175): S_x = max(|u|+a), S_y = max(|v|+a)
+// over the interior; "if (s > S) S = s" ignores NaN exactly as the reference.  The
+// max is exact, so the block order does not matter.
+__device__ __forceinline__ double nanmax(double a, double b) { return b > a ? b : a; }
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = nanmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U, Geom g, Phys ph,
+                                                    double* __restrict__ partial) {
+    const double gam = ph.gamma;
+    double sx = 0.0, sy = 0.0;
+    const int nj = g.C - 2 * GC;
+    const long total = (long)(g.R - 2 * GC) * nj;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = GC + (int)(t / nj), j = GC + (int)(t % nj);
+        const size_t c = (size_t)i * g.ld + j;
+        const double rho = U[c];
+        const double u = U[g.plane + c] / rho;
+        const double v = U[2 * g.plane + c] / rho;
+        const double p = (U[3 * g.plane + c] - 0.5 * rho * (u * u + v * v)) * (gam - 1.0);
+        const double a = sqrt(gam * p / rho);
+        const double lx = fabs(u) + a, ly = fabs(v) + a;
+        if (lx > sx) sx = lx;
+        if (ly > sy) sy = ly;
+    }
+    sx = wave_max(sx);
+    sy = wave_max(sy);
+    __shared__ double red[2][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][w] = sx;
+        red[1][w] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0][0], b = red[1][0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+            a = nanmax(a, red[0][k]);
+            b = nanmax(b, red[1][k]);
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// Reduces the per-block maxima and forms dt = CFL*min(dx,dy)/(S_x+S_y); optionally
+// clips it against the device clock as the driver does (src/RP_FVM.c:186-191):
+// clock[0] = t, clock[1] = t_stop.
+__global__ void k_dt_final(const double* __restrict__ partial, int nblocks, Phys ph,
+                           double* __restrict__ dt_out, const double* __restrict__ clock,
+                           unsigned* __restrict__ faults) {
+    double sx = 0.0, sy = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+        sx = nanmax(sx, partial[2 * k]);
+        sy = nanmax(sy, partial[2 * k + 1]);
+    }
+    sx = wave_max(sx);
+    sy = wave_max(sy);
+    if (threadIdx.x == 0) {
+        double dt = ph.cfl * min2(ph.dx, ph.dy) / (sx + sy);
+        if (clock) {
+            const double t = clock[0], ts = clock[1];
+            if (t + dt > ts) dt = ts - t;
+            if (t + dt >= ts) dt = ts - t;
+        }
+        if (!(dt > 0.0) || !isfinite(dt)) atomicOr(faults, 2u);
+        *dt_out = dt;
+    }
+}
+
+// ----------------------------------------------------------------- launchers
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& ph,
+                        unsigned* faults, hipStream_t s) {
+    const int nj = DIR == 1 ? g.C - 2 * GC : g.C - 2 * GC + 1;
+    const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
+    dim3 block(64, 4);
+    dim3 grid((nj + 63) / 64, (ni + 3) / 4);
+    hipLaunchKernelGGL((k_flux<DIR, ORDER, CHAR, SOLVER>), grid, block, 0, s, Y, F, g, ph, faults);
+}
+
+template <int ORDER, bool CHAR, int SOLVER>
+static FluxPair pick3() {
+    return FluxPair{&launch_flux<1, ORDER, CHAR, SOLVER>, &launch_flux<2, ORDER, CHAR, SOLVER>};
+}
+
+template <int ORDER, bool CHAR>
+static FluxPair pick2(int solver) {
+    switch (solver) {
+        case SOLVER_HLL: return pick3<ORDER, CHAR, SOLVER_HLL>();
+        case SOLVER_HLLC: return pick3<ORDER, CHAR, SOLVER_HLLC>();
+        case SOLVER_ROE: return pick3<ORDER, CHAR, SOLVER_ROE>();
+        default: return pick3<ORDER, CHAR, SOLVER_LLF>();
+    }
+}
+
+template <int ORDER>
+static FluxPair pick1(bool ch, int solver) {
+    return ch ? pick2<ORDER, true>(solver) : pick2<ORDER, false>(solver);
+}
+
+FluxPair select_flux(int order, bool characteristic, int solver) {
+    switch (order) {
+        case 1: return pick1<1>(characteristic, solver);
+        case 2: return pick1<2>(characteristic, solver);
+        case 3: return pick1<3>(characteristic, solver);
+        default: return pick1<5>(characteristic, solver);
+    }
+}
+
+void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st) {
+    const int n = s.hi - s.lo + 1;
+    if (n <= 0) return;
+    dim3 block(64);
+    dim3 grid((n + 63) / 64, 4);
+    hipLaunchKernelGGL(k_bc_side, grid, block, 0, st, U, g, s);
+}
+
+void launch_update(const double* Un, const double* Y, double* out, const double* F,
+                   const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                   hipStream_t s) {
+    const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
+    dim3 block(64, 4);
+    dim3 grid((nj + 63) / 64, (ni + 3) / 4);
+    hipLaunchKernelGGL(k_update, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind);
+}
+
+int dt_blocks(const Geom& g) {
+    const long cells = (long)(g.R - 2 * GC) * (g.C - 2 * GC);
+    long b = (cells + 255) / 256;
+    return (int)(b < 2048 ? (b < 1 ? 1 : b) : 2048);
+}
+
+void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,
+               const double* clock, unsigned* faults, hipStream_t s) {
+    const int nb = dt_blocks(g);
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(256), 0, s, U, g, ph, partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, partial, nb, ph, dt_out, clock, faults);
+}
+
+}  // namespace fvm

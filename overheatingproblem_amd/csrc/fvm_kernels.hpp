@@ -1,0 +1,51 @@
+// fvm_kernels.hpp -- launch interface between the C ABI (fvm_api.cpp) and the
+// gfx950 kernels (fvm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace fvm {
+
+constexpr int GC = 4;  // GhostCell, include/Golbal.h:49
+
+// Device grid geometry: planes of R rows x ld doubles; plane v at base + v*plane.
+struct Geom {
+    int R, C;       // rows (x, with ghosts) and logical columns (y, with ghosts)
+    int ld;         // row pitch in doubles (>= C)
+    size_t plane;   // plane stride in doubles
+};
+
+// Physical and scheme parameters read by the kernels.
+struct Phys {
+    double gamma, dx, dy, cfl, gravity;
+    int source, positivity;
+};
+
+enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4 };
+
+// One side of the ghost fill: side 0..3 = left, right, bottom, top; tangential index
+// range [lo, hi] (inclusive); value[] is the inflow or fixed state when used.
+struct BcSide {
+    int side, type, lo, hi;
+    double value[4];
+};
+
+enum : int { UPD_RESIDUAL = 0, UPD_EULER = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
+
+using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phys& ph,
+                            unsigned* faults, hipStream_t s);
+struct FluxPair {
+    FluxLaunch x, y;
+};
+
+FluxPair select_flux(int order, bool characteristic, int solver);
+void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st);
+void launch_update(const double* Un, const double* Y, double* out, const double* F,
+                   const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                   hipStream_t s);
+int dt_blocks(const Geom& g);
+void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,
+               const double* clock, unsigned* faults, hipStream_t s);
+
+}  // namespace fvm

@@ -1,0 +1,526 @@
+// fvm_math.hpp -- FP64 point math of the 2-D Euler FV step on gfx950.
+//
+// Every expression is written so that, compiled with -ffp-contract=off (no FMA
+// contraction) and IEEE division/sqrt, it rounds exactly like the reference's
+// x86-64 build (-O3, SSE2, no FMA).  Each function names the reference routine
+// whose arithmetic it reproduces.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fvm_kernels.hpp"
+
+namespace fvm {
+
+// ------------------------------------------------------------------- helpers
+// sgn / max_of_two / min_of_two / min_mod / van_leer: include/function.h:31-74
+__device__ __forceinline__ int sgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+__device__ __forceinline__ double max2(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double min2(double a, double b) { return a > b ? b : a; }
+__device__ __forceinline__ double minmod(double a, double b) {
+    return 0.5 * (sgn(a) + sgn(b)) * min2(fabs(a), fabs(b));
+}
+__device__ __forceinline__ double vanleer(double a, double b) {
+    return ((sgn(a) + sgn(b)) * a * b) / (fabs(a) + fabs(b) + 1.0e-6);
+}
+
+// ---------------------------------------------------- scalar reconstructions
+// Smoothness indicators shared by WENO5-JS and WENO5-Z (Jiang & Shu 1996),
+// include/Reconstruction.h:1426-1433 and :1537-1544.
+struct Beta5 {
+    double b1, b2, b3;
+};
+__device__ __forceinline__ Beta5 beta5(double v1, double v2, double v3, double v4, double v5) {
+    Beta5 r;
+    r.b1 = 13.0 / 12.0 * (v1 - 2.0 * v2 + v3) * (v1 - 2.0 * v2 + v3)
+         + 0.25 * (v1 - 4.0 * v2 + 3.0 * v3) * (v1 - 4.0 * v2 + 3.0 * v3);
+    r.b2 = 13.0 / 12.0 * (v2 - 2.0 * v3 + v4) * (v2 - 2.0 * v3 + v4)
+         + 0.25 * (v2 - v4) * (v2 - v4);
+    r.b3 = 13.0 / 12.0 * (v3 - 2.0 * v4 + v5) * (v3 - 2.0 * v4 + v5)
+         + 0.25 * (3.0 * v3 - 4.0 * v4 + v5) * (3.0 * v3 - 4.0 * v4 + v5);
+    return r;
+}
+
+// Weighted combination of the three third-order candidates, :1567-1569.
+__device__ __forceinline__ double weno5_combine(double a1, double a2, double a3, double v1,
+                                                double v2, double v3, double v4, double v5) {
+    const double w1 = a1 / (a1 + a2 + a3);
+    const double w2 = a2 / (a1 + a2 + a3);
+    const double w3 = a3 / (a1 + a2 + a3);
+    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0
+         + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0
+         + w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+}
+
+// WENO5-Z (Borges et al. 2008, exponent 1, eps 1e-15): WENO5Z_L/R,
+// include/Reconstruction.h:1519-1624.  (v1..v5) in upwind order.
+__device__ __forceinline__ double weno5z(double v1, double v2, double v3, double v4, double v5) {
+    const double eps = 1.0e-15;
+    const Beta5 b = beta5(v1, v2, v3, v4, v5);
+    const double tau = fabs(b.b1 - b.b3);
+    const double a1 = 0.1 * (1.0 + tau / (b.b1 + eps));
+    const double a2 = 0.6 * (1.0 + tau / (b.b2 + eps));
+    const double a3 = 0.3 * (1.0 + tau / (b.b3 + eps));
+    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5);
+}
+
+// WENO5-JS: WENO5_L/R, include/Reconstruction.h:1408-1508.
+__device__ __forceinline__ double weno5js(double v1, double v2, double v3, double v4, double v5) {
+    const double eps = 1.0e-15;
+    const Beta5 b = beta5(v1, v2, v3, v4, v5);
+    const double a1 = 0.1 / (eps + b.b1) / (eps + b.b1);
+    const double a2 = 0.6 / (eps + b.b2) / (eps + b.b2);
+    const double a3 = 0.3 / (eps + b.b3) / (eps + b.b3);
+    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5);
+}
+
+// WENO3_L/R, include/Reconstruction.h:1326-1398 (pow(x,2) folded to x*x).
+__device__ __forceinline__ double weno3(double v1, double v2, double v3) {
+    const double eps = 1e-6;
+    const double s1 = (v2 - v1) * (v2 - v1);
+    const double s2 = (v3 - v2) * (v3 - v2);
+    const double a1 = (1.0 / 3.0) / ((eps + s1) * (eps + s1));
+    const double a2 = (2.0 / 3.0) / ((eps + s2) * (eps + s2));
+    const double w1 = a1 / (a1 + a2);
+    const double w2 = a2 / (a1 + a2);
+    return w1 * (-0.5 * v1 + 1.5 * v2) + w2 * (0.5 * v2 + 0.5 * v3);
+}
+
+// TVD slopes, include/Reconstruction.h:1088-1209: minmod for the component branch,
+// van Leer for the characteristic branch.
+template <bool VANLEER>
+__device__ __forceinline__ double tvd_slope(double v1, double v2, double v3, double delta) {
+    const double s1 = (v2 - v1) / delta;
+    const double s2 = (v3 - v2) / delta;
+    return VANLEER ? vanleer(s1, s2) : minmod(s1, s2);
+}
+
+// Face values of one scalar stencil s[0..5] (cells i-2..i+3) for reconstruction
+// order ORDER; CHAR selects WENO5-Z / van Leer (characteristic branch) over
+// WENO5-JS / minmod (component branch).
+template <int ORDER, bool CHAR>
+__device__ __forceinline__ void recon_scalar(const double s[6], double delta, double& left,
+                                             double& right) {
+    if constexpr (ORDER == 1) {
+        left = s[2];
+        right = s[3];
+    } else if constexpr (ORDER == 2) {
+        const double sl = tvd_slope<CHAR>(s[1], s[2], s[3], delta);
+        const double sr = tvd_slope<CHAR>(s[2], s[3], s[4], delta);
+        left = s[2] + 0.5 * sl * delta;
+        right = s[3] - 0.5 * sr * delta;
+    } else if constexpr (ORDER == 3) {
+        left = weno3(s[1], s[2], s[3]);
+        right = weno3(s[4], s[3], s[2]);
+    } else {
+        if constexpr (CHAR) {
+            left = weno5z(s[0], s[1], s[2], s[3], s[4]);
+            right = weno5z(s[5], s[4], s[3], s[2], s[1]);
+        } else {
+            left = weno5js(s[0], s[1], s[2], s[3], s[4]);
+            right = weno5js(s[5], s[4], s[3], s[2], s[1]);
+        }
+    }
+}
+
+// ------------------------------------------------------------ eigenvectors
+// Con_to_Pri_2D for one cell, include/initialize.h:1443-1451.
+__device__ __forceinline__ void primitive(double g, const double c[4], double p[4]) {
+    const double rho = c[0];
+    const double u = c[1] / c[0];
+    const double v = c[2] / c[0];
+    p[0] = rho;
+    p[1] = u;
+    p[2] = v;
+    p[3] = (c[3] - 0.5 * rho * (u * u + v * v)) * (g - 1.0);
+}
+
+// The left (EL[wave][var]) and right (ER[var][wave]) eigenvectors of the face between
+// primitive states a and b.  DIR 1: Compute_Eigen_2DX (include/Characteriz.h:300-398);
+// DIR 2: Compute_Eigen_2DY (:402-500), with its literal 0.0/c and -0.0 terms.
+template <int DIR>
+__device__ __forceinline__ void face_eigen(double g, const double a[4], const double b[4],
+                                           double EL[4][4], double ER[4][4]) {
+    const double Ha = 0.5 * (a[1] * a[1] + a[2] * a[2]) + (g / (g - 1.0)) * (a[3] / a[0]);
+    const double Hb = 0.5 * (b[1] * b[1] + b[2] * b[2]) + (g / (g - 1.0)) * (b[3] / b[0]);
+    const double D = sqrt(b[0] / a[0]);
+    const double D1 = D + 1.0;
+    const double u = (a[1] + D * b[1]) / D1;
+    const double v = (a[2] + D * b[2]) / D1;
+    const double H = (Ha + D * Hb) / D1;
+    const double q2 = u * u + v * v;
+    const double c2 = (g - 1.0) * (H - 0.5 * q2);
+    const double c = sqrt(c2);
+    const double b1 = (g - 1.0) / c2;
+    const double b2 = 1.0 + b1 * q2 - b1 * H;
+    const double ic = 1.0 / c;
+    if constexpr (DIR == 1) {
+        EL[0][0] = 0.5 * (b2 + u / c);
+        EL[0][1] = -0.5 * (b1 * u + ic);
+        EL[0][2] = -0.5 * b1 * v;
+        EL[0][3] = 0.5 * b1;
+        EL[1][0] = -q2 + H;
+        EL[1][1] = u;
+        EL[1][2] = v;
+        EL[1][3] = -1.0;
+        EL[2][0] = v;
+        EL[2][1] = 0.0;
+        EL[2][2] = -1.0;
+        EL[2][3] = 0.0;
+        EL[3][0] = 0.5 * (b2 - u / c);
+        EL[3][1] = 0.5 * (-b1 * u + ic);
+        EL[3][2] = 0.5 * -b1 * v;
+        EL[3][3] = 0.5 * b1;
+        ER[0][0] = 1.0;
+        ER[0][1] = b1;
+        ER[0][2] = 0.0;
+        ER[0][3] = 1.0;
+        ER[1][0] = u - 1.0 * c;
+        ER[1][1] = u * b1;
+        ER[1][2] = 0.0;
+        ER[1][3] = u + 1.0 * c;
+        ER[2][0] = v - 0.0 * c;
+        ER[2][1] = v * b1;
+        ER[2][2] = -1.0;
+        ER[2][3] = v + 0.0 * c;
+        ER[3][0] = H - u * c;
+        ER[3][1] = H * b1 - 1.0;
+        ER[3][2] = -v;
+        ER[3][3] = H + u * c;
+    } else {
+        const double zc = 0.0 / c;
+        EL[0][0] = 0.5 * (b2 + v / c);
+        EL[0][1] = -0.5 * (b1 * u + zc);
+        EL[0][2] = -0.5 * (b1 * v + ic);
+        EL[0][3] = 0.5 * b1;
+        EL[1][0] = -q2 + H;
+        EL[1][1] = u;
+        EL[1][2] = v;
+        EL[1][3] = -1.0;
+        EL[2][0] = -u;
+        EL[2][1] = 1.0;
+        EL[2][2] = -0.0;
+        EL[2][3] = 0.0;
+        EL[3][0] = 0.5 * (b2 - v / c);
+        EL[3][1] = 0.5 * (-b1 * u + zc);
+        EL[3][2] = 0.5 * (-b1 * v + ic);
+        EL[3][3] = 0.5 * b1;
+        ER[0][0] = 1.0;
+        ER[0][1] = b1;
+        ER[0][2] = 0.0;
+        ER[0][3] = 1.0;
+        ER[1][0] = u - 0.0 * c;
+        ER[1][1] = u * b1;
+        ER[1][2] = 1.0;
+        ER[1][3] = u + 0.0 * c;
+        ER[2][0] = v - 1.0 * c;
+        ER[2][1] = v * b1;
+        ER[2][2] = -0.0;
+        ER[2][3] = v + 1.0 * c;
+        ER[3][0] = H - v * c;
+        ER[3][1] = H * b1 - 1.0;
+        ER[3][2] = u;
+        ER[3][3] = H + v * c;
+    }
+}
+
+// Face states from the six-cell stencil st[n][var].  Characteristic branch:
+// project w = L.U per wave (sum from 0.0 in var order), reconstruct, back-project
+// U = R.w (sum from 0.0 in wave order): include/Reconstruction.h:909-967 and its
+// Godunov/TVD/WENO3 twins; component branch: :1031-1073 and twins.
+template <int DIR, int ORDER, bool CHAR>
+__device__ __forceinline__ void reconstruct_face(double g, double delta, const double st[6][4],
+                                                 double UL[4], double UR[4]) {
+    if constexpr (CHAR) {
+        double pa[4], pb[4], EL[4][4], ER[4][4];
+        primitive(g, st[2], pa);
+        primitive(g, st[3], pb);
+        face_eigen<DIR>(g, pa, pb, EL, ER);
+        double wl[4], wr[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if constexpr (ORDER == 1) {
+                double l = 0.0, r = 0.0;
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    l += st[2][m] * EL[k][m];
+                    r += st[3][m] * EL[k][m];
+                }
+                wl[k] = l;
+                wr[k] = r;
+            } else {
+                double w[6];
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) acc += st[n][m] * EL[k][m];
+                    w[n] = acc;
+                }
+                recon_scalar<ORDER, true>(w, delta, wl[k], wr[k]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double l = 0.0, r = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                l += wl[k] * ER[v][k];
+                r += wr[k] * ER[v][k];
+            }
+            UL[v] = l;
+            UR[v] = r;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double s[6] = {st[0][v], st[1][v], st[2][v], st[3][v], st[4][v], st[5][v]};
+            recon_scalar<ORDER, false>(s, delta, UL[v], UR[v]);
+        }
+    }
+}
+
+// ------------------------------------------------------------ Riemann solvers
+struct Side {
+    double r, ru, rv, E, ir, u, v, u2, v2, p;
+};
+
+__device__ __forceinline__ Side make_side(double g, const double c[4]) {
+    Side s;
+    s.r = c[0];
+    s.ru = c[1];
+    s.rv = c[2];
+    s.E = c[3];
+    s.ir = 1.0 / s.r;
+    s.u = s.ru * s.ir;
+    s.v = s.rv * s.ir;
+    s.u2 = s.u * s.u;
+    s.v2 = s.v * s.v;
+    s.p = (s.E - 0.5 * s.r * (s.u2 + s.v2)) * (g - 1.0);
+    return s;
+}
+
+template <int DIR>
+__device__ __forceinline__ void phys_flux(const Side& s, double H, double F[4]) {
+    if constexpr (DIR == 1) {
+        F[0] = s.r * s.u;
+        F[1] = s.r * s.u2 + s.p;
+        F[2] = s.r * s.v * s.u;
+        F[3] = s.r * H * s.u;
+    } else {
+        F[0] = s.r * s.v;
+        F[1] = s.r * s.u * s.v;
+        F[2] = s.r * s.v2 + s.p;
+        F[3] = s.r * H * s.v;
+    }
+}
+
+// HLLC with Einfeldt speeds, include/scheme.h:690-1001 (per-direction round-off:
+// sound speed :745-746 vs :895-896, Einfeldt eta :777 vs :929).
+template <int DIR>
+__device__ __forceinline__ int hllc(double g, const double cl[4], const double cr[4], double F[4]) {
+    const double gm1 = g - 1.0;
+    const double gr = g / gm1;
+    const Side L = make_side(g, cl), R = make_side(g, cr);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double aL, aR;
+    if constexpr (DIR == 1) {
+        aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
+        aR = sqrt((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
+    } else {
+        aL = sqrt(g * L.p * L.ir);
+        aR = sqrt(g * R.p * R.ir);
+    }
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double ss = sL + sR;
+    const double iss = 1.0 / ss;
+    double eta;
+    if constexpr (DIR == 1) eta = 0.5 * sL * sR * iss * iss;
+    else eta = 0.5 * sL * sR / (ss * ss);
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const double nbar = ((DIR == 1 ? sL * L.u + sR * R.u : sL * L.v + sR * R.v)) * iss;
+    const double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    const double sl = nbar - d, sr = nbar + d;
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+        return 0;
+    }
+    if (sr <= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+        return 0;
+    }
+    const double denom = L.r * (sl - qL) - R.r * (sr - qR);
+    const double sstar = (R.p - L.p + L.r * qL * (sl - qL) - R.r * qR * (sr - qR)) / denom;
+    if (sstar >= 0.0) {
+        const double m = L.r * (sl - qL) / (sl - sstar);
+        const double e = L.E * L.ir + (sstar - qL) * (sstar + L.p / (L.r * (sl - qL)));
+        F[0] = FL[0] + sl * (m - L.r);
+        if constexpr (DIR == 1) {
+            F[1] = FL[1] + sl * (m * sstar - L.ru);
+            F[2] = FL[2] + sl * (m * L.v - L.rv);
+        } else {
+            F[1] = FL[1] + sl * (m * L.u - L.ru);
+            F[2] = FL[2] + sl * (m * sstar - L.rv);
+        }
+        F[3] = FL[3] + sl * (m * e - L.E);
+    } else {
+        const double m = R.r * (sr - qR) / (sr - sstar);
+        const double e = R.E * R.ir + (sstar - qR) * (sstar + R.p / (R.r * (sr - qR)));
+        F[0] = FR[0] + sr * (m - R.r);
+        if constexpr (DIR == 1) {
+            F[1] = FR[1] + sr * (m * sstar - R.ru);
+            F[2] = FR[2] + sr * (m * R.v - R.rv);
+        } else {
+            F[1] = FR[1] + sr * (m * R.u - R.ru);
+            F[2] = FR[2] + sr * (m * sstar - R.rv);
+        }
+        F[3] = FR[3] + sr * (m * e - R.E);
+    }
+    return 0;
+}
+
+// HLL with Einfeldt speeds, include/scheme.h:400-674.  The reference prints and
+// skips a face with rho <= 0, NaN rho (:427-436) or a NaN speed (:500-503), which
+// leaves stale memory in its flux array; here such a face gets NaN and a fault bit.
+template <int DIR>
+__device__ __forceinline__ int hll(double g, const double cl[4], const double cr[4], double F[4]) {
+    if (cl[0] <= 0.0 || cr[0] <= 0.0 || isnan(cl[0]) || isnan(cr[0])) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
+        return 1;
+    }
+    const Side L = make_side(g, cl), R = make_side(g, cr);
+    const double gr = g / (g - 1.0);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    const double aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
+    const double aR = sqrt((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double ss = sL + sR;
+    const double iss = 1.0 / ss;
+    const double eta = 0.5 * sL * sR / (ss * ss);
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const double nbar = ((DIR == 1 ? sL * L.u + sR * R.u : sL * L.v + sR * R.v)) * iss;
+    const double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    if (isnan(d)) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
+        return 1;
+    }
+    const double sl = nbar - d, sr = nbar + d;
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sr <= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    } else {
+        const double isd = 1.0 / (sr - sl);
+        const double slsr = sl * sr;
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = (sr * FL[k] - sl * FR[k] + slsr * (cr[k] - cl[k])) * isd;
+    }
+    return 0;
+}
+
+// Roe with the reference's no-op entropy fix, include/scheme.h:1018-1322.
+template <int DIR>
+__device__ __forceinline__ int roe(double g, const double cl[4], const double cr[4], double F[4]) {
+    const double gm1 = g - 1.0;
+    const double gr = g / gm1;
+    const Side L = make_side(g, cl), R = make_side(g, cr);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double ss = sL + sR;
+    const double iss = 1.0 / ss;
+    const double rbar = sL * sR;
+    const double ubar = (sL * L.u + sR * R.u) * iss;
+    const double vbar = (sL * L.v + sR * R.v) * iss;
+    const double Hbar = (sL * HL + sR * HR) * iss;
+    const double q2 = ubar * ubar + vbar * vbar;
+    const double cbar = sqrt(gm1 * (Hbar - 0.5 * q2));
+    const double nbar = DIR == 1 ? ubar : vbar;
+    const double lam[4] = {fabs(nbar - cbar), fabs(nbar), fabs(nbar), fabs(nbar + cbar)};
+    const double dr = R.r - L.r, du = R.u - L.u, dv = R.v - L.v, dp = R.p - L.p;
+    const double cs = cbar * cbar;
+    const double i2cs = 1.0 / (2.0 * cs);
+    const double ics = 1.0 / cs;
+    const double dn = DIR == 1 ? du : dv, dtg = DIR == 1 ? dv : du;
+    const double al[4] = {(dp - rbar * cbar * dn) * i2cs, dr - dp * ics, rbar * dtg,
+                          (dp + rbar * cbar * dn) * i2cs};
+    double ev[4][4];
+    if constexpr (DIR == 1) {
+        ev[0][0] = 1.0; ev[0][1] = ubar - cbar; ev[0][2] = vbar; ev[0][3] = Hbar - ubar * cbar;
+        ev[1][0] = 1.0; ev[1][1] = ubar;        ev[1][2] = vbar; ev[1][3] = 0.5 * q2;
+        ev[2][0] = 0.0; ev[2][1] = 0.0;         ev[2][2] = 1.0;  ev[2][3] = vbar;
+        ev[3][0] = 1.0; ev[3][1] = ubar + cbar; ev[3][2] = vbar; ev[3][3] = Hbar + ubar * cbar;
+    } else {
+        ev[0][0] = 1.0; ev[0][1] = ubar; ev[0][2] = vbar - cbar; ev[0][3] = Hbar - vbar * cbar;
+        ev[1][0] = 1.0; ev[1][1] = ubar; ev[1][2] = vbar;        ev[1][3] = 0.5 * q2;
+        ev[2][0] = 0.0; ev[2][1] = 1.0;  ev[2][2] = 0.0;         ev[2][3] = ubar;
+        ev[3][0] = 1.0; ev[3][1] = ubar; ev[3][2] = vbar + cbar; ev[3][3] = Hbar + vbar * cbar;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]);
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const double term = 0.5 * lam[w] * al[w];
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= term * ev[w][k];
+    }
+    return 0;
+}
+
+// Local Lax-Friedrichs (the reference's default solver branch), include/scheme.h:229-381.
+template <int DIR>
+__device__ __forceinline__ int llf(double g, const double cl[4], const double cr[4], double F[4]) {
+    const double gm1 = g - 1.0;
+    const Side L = make_side(g, cl), R = make_side(g, cr);
+    const double aL = sqrt(g * L.p * L.ir);
+    const double aR = sqrt(g * R.p * R.ir);
+    const double gr = g / gm1;
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const double s = max2(fabs(qL) + aL, fabs(qR) + aR);
+#pragma unroll
+    for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]) - 0.5 * s * (cr[k] - cl[k]);
+    return 0;
+}
+
+enum : int { SOLVER_HLL = 1, SOLVER_HLLC = 2, SOLVER_ROE = 3, SOLVER_LLF = 10 };
+
+template <int DIR, int SOLVER>
+__device__ __forceinline__ int riemann(double g, const double cl[4], const double cr[4], double F[4]) {
+    if constexpr (SOLVER == SOLVER_HLL) return hll<DIR>(g, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLC) return hllc<DIR>(g, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_ROE) return roe<DIR>(g, cl, cr, F);
+    else return llf<DIR>(g, cl, cr, F);
+}
+
+// Positivity test of a face state (rho > 0, p > 0, finite); used only by the
+// optional positivity safeguard, which the reference does not have.
+__device__ __forceinline__ bool admissible(double g, const double c[4]) {
+    const double ir = 1.0 / c[0];
+    const double u = c[1] * ir, v = c[2] * ir;
+    const double p = (c[3] - 0.5 * c[0] * (u * u + v * v)) * (g - 1.0);
+    return c[0] > 0.0 && p > 0.0 && isfinite(p) && isfinite(u) && isfinite(v);
+}
+
+}  // namespace fvm

@@ -1,0 +1,275 @@
+"""Write the golden fixtures under tests/golden/ from the reference's own code.
+
+Runs oracle/_ref/ref_harness (the reference solver's headers compiled with the
+driver in oracle/ref_harness.c; build it with ``make -C oracle ref``) on seeded
+inputs and stores inputs and outputs as .npz data.  Run in the build container,
+where the reference sources exist:
+
+    python tests/golden/make_golden.py
+
+The fixtures are data only (inputs, outputs, the configuration as JSON).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.pyoracle import harness_job  # noqa: E402
+from overheatingproblem_amd import cases  # noqa: E402
+from overheatingproblem_amd.config import make_config  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+job_text = harness_job
+
+
+def run(job, inputs, n_out):
+    with tempfile.TemporaryDirectory() as td:
+        jp, ip, op = (os.path.join(td, n) for n in ("job.txt", "in.bin", "out.bin"))
+        with open(jp, "w") as f:
+            f.write(job)
+        np.concatenate([np.ravel(a) for a in inputs]).astype("<f8").tofile(ip)
+        env = dict(os.environ, OMP_NUM_THREADS="4")
+        subprocess.run([HARNESS, jp, ip, op], check=True, env=env, stdout=subprocess.DEVNULL)
+        out = np.fromfile(op, dtype="<f8")
+    assert out.size == n_out, (out.size, n_out)
+    return out
+
+
+def cfg_json(cfg):
+    return np.array(json.dumps(cfg.as_dict()))
+
+
+# -------------------------------------------------------------------------- fixtures
+def make_weno():
+    rng = np.random.default_rng(1001)
+    parts = []
+    x = np.linspace(-2, 2, 6)
+    for _ in range(400):  # smooth polynomials / sines at several scales
+        a = rng.normal(size=4)
+        scale = 10.0 ** rng.uniform(-8, 3)
+        parts.append(scale * (a[0] + a[1] * x + a[2] * x ** 2 + a[3] * np.sin(x)))
+    for _ in range(400):  # jumps
+        lo, hi = rng.uniform(-3, 3, size=2)
+        cut = rng.integers(1, 6)
+        parts.append(np.where(np.arange(6) < cut, lo, hi))
+    for _ in range(200):  # noise
+        parts.append(rng.normal(size=6) * 10.0 ** rng.uniform(-3, 3))
+    parts.append(np.zeros(6))
+    parts.append(np.full(6, 1.7))
+    parts.append(np.full(6, -3.25e-7))
+    parts.append(np.array([0, 0, 0, 1, 1, 1.0]))
+    stencils = np.array(parts, dtype=np.float64)
+    out = run(job_text("weno", n=len(stencils)), [stencils], 4 * len(stencils)).reshape(-1, 4)
+    return {"stencils": stencils, "out": out}
+
+
+def physical_pairs(rng, shape, strong=True):
+    """Random primitive states -> conserved, for Riemann-solver fixtures."""
+    rho = rng.uniform(0.1, 5.0, size=shape)
+    u = rng.uniform(-3.0, 3.0, size=shape)
+    v = rng.uniform(-3.0, 3.0, size=shape)
+    p = rng.uniform(0.05, 10.0, size=shape)
+    return rho, u, v, p
+
+
+def make_flux():
+    out = {}
+    nx, ny = 12, 10
+    R, C = nx + 8, ny + 8
+    rng = np.random.default_rng(2002)
+    for scheme in (1, 2, 3, 10):
+        for gamma in (1.4, 5.0 / 3.0):
+            cfg = make_config(nx, ny, scheme=scheme, gamma=gamma)
+            rl, ul, vl, pl = physical_pairs(rng, (R, C))
+            Ls = cases.prim_to_cons(rl, ul, vl, pl, gamma)
+            # half of the faces get a nearby right state, half an independent one
+            rr, ur, vr, pr = physical_pairs(rng, (R, C))
+            near = rng.uniform(size=(R, C)) < 0.5
+            jit = lambda a: a * (1.0 + 0.01 * rng.uniform(-1, 1, size=(R, C)))
+            rr = np.where(near, jit(rl), rr)
+            ur = np.where(near, ul + 0.01 * rng.uniform(-1, 1, size=(R, C)), ur)
+            vr = np.where(near, vl + 0.01 * rng.uniform(-1, 1, size=(R, C)), vr)
+            pr = np.where(near, jit(pl), pr)
+            Rs = cases.prim_to_cons(rr, ur, vr, pr, gamma)
+            n = 4 * R * C
+            res = run(job_text("flux", cfg), [Ls, Rs], 2 * n)
+            key = f"s{scheme}_g{'53' if gamma > 1.5 else '14'}"
+            out[key + "_cfg"] = cfg_json(cfg)
+            out[key + "_L"] = Ls
+            out[key + "_R"] = Rs
+            out[key + "_F1"] = res[:n].reshape(4, R, C)
+            out[key + "_F2"] = res[n:].reshape(4, R, C)
+    return out
+
+
+def blocky_state(nx, ny, seed, gamma=1.4):
+    """Smooth random state with a superposed oblique discontinuity."""
+    U = cases.random_smooth_state(nx, ny, seed, gamma=gamma)
+    rho, u, v, p = cases.cons_to_prim(U, gamma)
+    X, Y = cases.cell_centres(nx, ny, 1.0 / nx, 1.0 / ny)
+    jump = X + 0.7 * Y > 0.8
+    rho = np.where(jump, rho * 2.5, rho)
+    p = np.where(jump, p * 3.0, p)
+    return cases.prim_to_cons(rho, u, v, p, gamma)
+
+
+def make_eigen():
+    nx, ny = 10, 8
+    cfg = make_config(nx, ny, gamma=1.4)
+    U = blocky_state(nx, ny, 3003)
+    n = 16 * cfg.R * cfg.C
+    res = run(job_text("eigen", cfg), [U], 4 * n)
+    shp = (4, 4, cfg.R, cfg.C)
+    return {"cfg": cfg_json(cfg), "U": U, "ELx": res[:n].reshape(shp), "ERx": res[n:2 * n].reshape(shp),
+            "ELy": res[2 * n:3 * n].reshape(shp), "ERy": res[3 * n:].reshape(shp)}
+
+
+def make_recon():
+    out = {}
+    nx, ny = 14, 12
+    for order in (1, 2, 3, 5):
+        for char in (0, 1):
+            cfg = make_config(nx, ny, recon_order=order, characteristic=char)
+            U = blocky_state(nx, ny, 4000 + 10 * order + char)
+            n = 4 * cfg.R * cfg.C
+            res = run(job_text("recon", cfg), [U], 4 * n).reshape(4, 4, cfg.R, cfg.C)
+            key = f"r{order}c{char}"
+            out[key + "_cfg"] = cfg_json(cfg)
+            out[key + "_U"] = U
+            out[key + "_CL1"], out[key + "_CR1"], out[key + "_CL2"], out[key + "_CR2"] = res
+    return out
+
+
+def make_bc():
+    out = {}
+    combos = {
+        "outflow": ((0, 0, 0, 0), 9, 7),
+        "reflection": ((1, 1, 1, 1), 9, 7),
+        "periodic": ((2, 2, 2, 2), 9, 7),
+        "inflow": ((3, 3, 3, 3), 9, 7),
+        "fixed": ((4, 4, 4, 4), 9, 7),
+        "noh": ((1, 0, 1, 0), 8, 8),
+        "dmr": ((4, 0, 1, 4), 10, 6),
+        "mixed": ((2, 3, 0, 1), 7, 9),
+        "tiny_periodic": ((2, 2, 2, 2), 2, 3),
+        "tiny_reflection": ((1, 1, 1, 1), 3, 1),
+        "tiny_outflow": ((0, 0, 0, 0), 1, 2),
+    }
+    rng = np.random.default_rng(5005)
+    for name, (bc, nx, ny) in combos.items():
+        inflow = rng.uniform(0.5, 2.0, size=4)
+        fixed = rng.uniform(0.5, 2.0, size=(4, 4))
+        cfg = make_config(nx, ny, bc=bc, inflow_state=inflow, fixed_state=fixed)
+        U = rng.uniform(-1.0, 2.0, size=cfg.shape)
+        res = run(job_text("bc", cfg), [U], U.size).reshape(cfg.shape)
+        out[name + "_cfg"] = cfg_json(cfg)
+        out[name + "_U"] = U
+        out[name + "_out"] = res
+    return out
+
+
+def ll_small(nx, ny, **kw):
+    return cases.four_quadrant(nx, ny, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3,
+                               seed=nx * 131 + ny, **kw)
+
+
+def make_residual():
+    out = {}
+    nx, ny = 16, 12
+    specs = []
+    for scheme in (1, 2, 3, 10):
+        for char in (1, 0):
+            specs.append((f"s{scheme}c{char}", dict(scheme=scheme, characteristic=char)))
+    specs.append(("src", dict(scheme=2, characteristic=1, source=True, gravity=-1.0,
+                              bc=(1, 1, 1, 1))))
+    for key, kw in specs:
+        cfg, U = ll_small(nx, ny, **kw)
+        n = U.size
+        res = run(job_text("residual", cfg), [U], 4 * n).reshape(4, 4, cfg.R, cfg.C)
+        out[key + "_cfg"] = cfg_json(cfg)
+        out[key + "_U0"] = U
+        out[key + "_Ubc"], out[key + "_L"], out[key + "_F"], out[key + "_G"] = res
+    return out
+
+
+def make_steps():
+    out = {}
+    specs = []
+    # solver x reconstruction-mode matrix on a perturbed Lax-Liu configuration
+    for scheme in (2, 1, 3, 10):
+        for char in (1, 0):
+            specs.append((f"ll_s{scheme}_c{char}",
+                          lambda s=scheme, c=char: ll_small(20, 16, scheme=s, characteristic=c), 5))
+    # time orders
+    for order in (1, 2):
+        specs.append((f"ll_rk{order}",
+                      lambda o=order: ll_small(20, 16, time_order=o, characteristic=1), 5))
+    # other reconstruction orders, both modes
+    for rec in (1, 2, 3):
+        for char in (1, 0):
+            specs.append((f"ll_r{rec}_c{char}",
+                          lambda r=rec, c=char: ll_small(20, 16, recon_order=r, characteristic=c), 5))
+    specs.append(("noh", lambda: cases.noh(16), 6))
+    specs.append(("noh_roe", lambda: cases.noh(16, scheme=3), 4))
+    specs.append(("sedov", lambda: cases.sedov(18), 5))
+    specs.append(("wave_periodic", lambda: cases.smooth_wave(16, 12, scheme=3), 5))
+    specs.append(("gravity", lambda: cases.four_quadrant(
+        12, 20, {"NE": (2, 0, 0, 2.5), "NW": (2, 0, 0, 2.5), "SW": (1, 0, 0, 2.5), "SE": (1, 0, 0, 2.5)},
+        perturb=1e-2, seed=7, source=True, gravity=-1.0, bc=(1, 1, 1, 1), characteristic=1), 5))
+    specs.append(("dmr", lambda: cases.double_mach(24, 8, characteristic=1), 4))
+    specs.append(("dmr_hllc", lambda: cases.double_mach(24, 8, scheme=2, characteristic=0), 4))
+    specs.append(("inflow", lambda: cases.four_quadrant(
+        16, 10, cases.LAX_LIU_3, perturb=1e-3, seed=3, bc=(3, 0, 2, 2)), 4))
+    specs.append(("fixed_dt", lambda: ll_small(12, 12, characteristic=1), 3))
+    for key, builder, steps in specs:
+        cfg, U = builder()
+        extra = {"steps": steps}
+        if key == "fixed_dt":
+            extra["fixed_dt"] = 1.0e-3
+        res = run(job_text("step", cfg, **extra), [U], steps + U.size)
+        out[key + "_cfg"] = cfg_json(cfg)
+        out[key + "_U0"] = U
+        out[key + "_dts"] = res[:steps]
+        out[key + "_U"] = res[steps:].reshape(cfg.shape)
+        if "fixed_dt" in extra:
+            out[key + "_fixed_dt"] = np.array(extra["fixed_dt"])
+    return out
+
+
+def make_dt():
+    out = {}
+    for key, (cfg, U) in {"ll": ll_small(20, 16), "noh": cases.noh(16), "sedov": cases.sedov(18),
+                          "dmr": cases.double_mach(24, 8)}.items():
+        res = run(job_text("dt", cfg), [U], 1)
+        out[key + "_cfg"] = cfg_json(cfg)
+        out[key + "_U"] = U
+        out[key + "_dt"] = res
+    return out
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
+    for name, fn in (("weno", make_weno), ("flux", make_flux), ("eigen", make_eigen),
+                     ("recon", make_recon), ("bc", make_bc), ("residual", make_residual),
+                     ("steps", make_steps), ("dt", make_dt)):
+        data = fn()
+        path = os.path.join(OUT, f"{name}.npz")
+        np.savez_compressed(path, **data)
+        print(f"{path}: {len(data)} arrays, {os.path.getsize(path) / 1024:.1f} KiB")
+
+
+if __name__ == "__main__":
+    main()

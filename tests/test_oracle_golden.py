@@ -1,0 +1,109 @@
+"""Pin the CPU oracle bit-for-bit against fixtures written by the reference's own code.
+
+The fixtures (tests/golden/*.npz) come from tests/golden/make_golden.py, which runs the
+reference solver's headers through oracle/ref_harness.c.  Equality is exact (+0 == -0,
+NaN matches NaN), over the index ranges the reference actually computes.
+"""
+import numpy as np
+import pytest
+
+from golden_util import cfg as golden_cfg, keys_with_suffix, load, same
+
+GC = 4
+
+
+def xfaces(c):
+    return (slice(None), slice(GC - 1, c.R - GC), slice(GC, c.C - GC))
+
+
+def yfaces(c):
+    return (slice(None), slice(GC, c.R - GC), slice(GC - 1, c.C - GC))
+
+
+def inner(c):
+    return (slice(None), slice(GC, c.R - GC), slice(GC, c.C - GC))
+
+
+def test_weno_points(oracle):
+    d = load("weno")
+    out = oracle.weno5_points(d["stencils"])
+    assert same(out, d["out"])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("flux"), "_cfg"))
+def test_riemann(oracle, key):
+    d = load("flux")
+    c = golden_cfg(d, key + "_cfg")
+    F1, _ = oracle.riemann(c, 1, d[key + "_L"], d[key + "_R"])
+    F2, _ = oracle.riemann(c, 2, d[key + "_L"], d[key + "_R"])
+    assert same(F1[xfaces(c)], d[key + "_F1"][xfaces(c)])
+    assert same(F2[yfaces(c)], d[key + "_F2"][yfaces(c)])
+
+
+def test_eigen(oracle):
+    d = load("eigen")
+    c = golden_cfg(d, "cfg")
+    U = np.ascontiguousarray(d["U"])
+    ELx, ERx = oracle.eigen(c, U, 1)
+    ELy, ERy = oracle.eigen(c, U, 2)
+    sx = (slice(None), slice(None), slice(GC - 1, c.R - GC), slice(GC, c.C - GC))
+    sy = (slice(None), slice(None), slice(GC, c.R - GC), slice(GC - 1, c.C - GC))
+    assert same(ELx[sx], d["ELx"][sx]) and same(ERx[sx], d["ERx"][sx])
+    assert same(ELy[sy], d["ELy"][sy]) and same(ERy[sy], d["ERy"][sy])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("recon"), "_cfg"))
+def test_reconstruction(oracle, key):
+    d = load("recon")
+    c = golden_cfg(d, key + "_cfg")
+    U = np.ascontiguousarray(d[key + "_U"])
+    CL1, CR1 = oracle.reconstruct(c, U, 1)
+    CL2, CR2 = oracle.reconstruct(c, U, 2)
+    assert same(CL1[xfaces(c)], d[key + "_CL1"][xfaces(c)])
+    assert same(CR1[xfaces(c)], d[key + "_CR1"][xfaces(c)])
+    assert same(CL2[yfaces(c)], d[key + "_CL2"][yfaces(c)])
+    assert same(CR2[yfaces(c)], d[key + "_CR2"][yfaces(c)])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("bc"), "_cfg"))
+def test_boundary_conditions(oracle, key):
+    d = load("bc")
+    c = golden_cfg(d, key + "_cfg")
+    U = np.array(d[key + "_U"])
+    oracle.boundary_conditions(c, U)
+    assert same(U, d[key + "_out"])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("residual"), "_cfg"))
+def test_space_operator(oracle, key):
+    d = load("residual")
+    c = golden_cfg(d, key + "_cfg")
+    U = np.array(d[key + "_U0"])
+    L, F, G, _ = oracle.space_operator(c, U)
+    assert same(U, d[key + "_Ubc"])
+    assert same(L[inner(c)], d[key + "_L"][inner(c)])
+    assert same(F[xfaces(c)], d[key + "_F"][xfaces(c)])
+    assert same(G[yfaces(c)], d[key + "_G"][yfaces(c)])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("steps"), "_cfg"))
+def test_steps(oracle, key):
+    d = load("steps")
+    c = golden_cfg(d, key + "_cfg")
+    U = np.array(d[key + "_U0"])
+    ref_dts = d[key + "_dts"]
+    dts = []
+    for s in range(len(ref_dts)):
+        dt = float(d[key + "_fixed_dt"]) if key + "_fixed_dt" in d.files else oracle.delta_t(c, U)
+        dts.append(dt)
+        oracle.time_advance(c, U, dt)
+    assert same(np.array(dts), ref_dts)
+    # whole array, ghost layers included
+    assert same(U, d[key + "_U"])
+
+
+@pytest.mark.parametrize("key", keys_with_suffix(load("dt"), "_cfg"))
+def test_delta_t(oracle, key):
+    d = load("dt")
+    c = golden_cfg(d, key + "_cfg")
+    assert oracle.delta_t(c, d[key + "_U"]) == float(d[key + "_dt"][0])
