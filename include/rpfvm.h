@@ -55,8 +55,10 @@ enum {
     RPFVM_FAULT_HLL_BAD_STATE = 1,   /* HLL face with rho <= 0, NaN rho or NaN wave speed
                                         (the reference prints and skips: scheme.h:427-436,500-503) */
     RPFVM_FAULT_NONFINITE_DT = 2,    /* dt reduction produced a non-finite or non-positive dt */
-    RPFVM_FAULT_MIXED_FALLBACK = 4   /* mixed form recomputed at least one face in FP64
+    RPFVM_FAULT_MIXED_FALLBACK = 4,  /* mixed form recomputed at least one face in FP64
                                         (informational, not an error) */
+    RPFVM_FAULT_POSITIVITY = 8       /* the positivity safeguard left a cell with rho <= 0
+                                        or p <= 0 */
 };
 
 /* Error codes (< 0). */

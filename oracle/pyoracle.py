@@ -43,6 +43,7 @@ def lib():
         L.orc_weno5_points.argtypes = [ctypes.c_int, dp, dp]
         L.orc_eigen.argtypes = [vp, dp, ctypes.c_int, dp, dp]
         L.orc_threads.restype = ctypes.c_int
+        L.orc_last_repairs.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -138,6 +139,11 @@ def eigen(cfg, U, direction):
 
 def threads():
     return int(lib().orc_threads())
+
+
+def last_repairs():
+    """Cells the positivity safeguard repaired during the last time_advance."""
+    return int(lib().orc_last_repairs())
 
 
 # ------------------------------------------------------------ reference harness

@@ -479,6 +479,47 @@ static int hll_(double g, int dir, const double cl[4], const double cr[4], doubl
     return 0;
 }
 
+/* Positively conservative HLLE (Einfeldt, Munz, Roe & Sjogreen 1991): wave-speed
+ * bounds s_L = min(q_L - a_L, q~ - c~), s_R = max(q_R + a_R, q~ + c~) with Roe averages.
+ * Used only by the positivity safeguard (an extension; the reference's HLL uses the
+ * unbounded speeds q~ -/+ d, which are not positivity preserving). */
+static int hlle_(double g, int dir, const double cl[4], const double cr[4], double F[4]) {
+    if (!(cl[0] > 0.0) || !(cr[0] > 0.0)) {
+        for (int k = 0; k < 4; k++) F[k] = NAN;
+        return 1;
+    }
+    side_t L, R;
+    side_(g, cl, &L);
+    side_(g, cr, &R);
+    const double gr = g / (g - 1.0);
+    double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double aL = sqrt(fmax(g * L.p * L.ir, 0.0));
+    double aR = sqrt(fmax(g * R.p * R.ir, 0.0));
+    double FL[4], FR[4];
+    phys_flux_(dir, &L, HL, FL);
+    phys_flux_(dir, &R, HR, FR);
+    double sL = sqrt(L.r), sR = sqrt(R.r);
+    double iss = 1.0 / (sL + sR);
+    double ubar = (sL * L.u + sR * R.u) * iss;
+    double vbar = (sL * L.v + sR * R.v) * iss;
+    double Hbar = (sL * HL + sR * HR) * iss;
+    double cbar = sqrt(fmax((g - 1.0) * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)), 0.0));
+    double qL = dir == 1 ? L.u : L.v, qR = dir == 1 ? R.u : R.v;
+    double qbar = dir == 1 ? ubar : vbar;
+    double sl = fmin(qL - aL, qbar - cbar);
+    double sr = fmax(qR + aR, qbar + cbar);
+    if (sl >= 0.0) {
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sr <= 0.0) {
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    } else {
+        double isd = 1.0 / (sr - sl);
+        for (int k = 0; k < 4; k++) F[k] = (sr * FL[k] - sl * FR[k] + sl * sr * (cr[k] - cl[k])) * isd;
+    }
+    return 0;
+}
+
 /* Roe with a no-op entropy fix: include/scheme.h:1018-1322. */
 static int roe_(double g, int dir, const double cl[4], const double cr[4], double F[4]) {
     const double gm1 = g - 1.0;
@@ -764,54 +805,166 @@ int orc_space_operator(const rpfvm_config *cfg, double *U, double *L, double *F,
 }
 
 /* --------------------------------------------------------------- time advance */
+enum { K_EULER = 1, K_RK2_2 = 2, K_RK3_2 = 3, K_RK3_3 = 4 };
+
+/* Stage combine of RK1/RK2/RK3 (include/time_advance.h:697, 757-777, 832-872). */
+static inline double combine_(int kind, double un, double y, double l, double dt) {
+    switch (kind) {
+        case K_EULER: return y + dt * l;
+        case K_RK2_2: return 0.5 * un + 0.5 * (y + dt * l);
+        case K_RK3_2: return (3.0 / 4.0) * un + (1.0 / 4.0) * (y + dt * l);
+        default: return (1.0 / 3.0) * un + (2.0 / 3.0) * (y + dt * l);
+    }
+}
+
+static long g_repairs = 0;  /* cells repaired by the positivity safeguard, last step */
+
+long orc_last_repairs(void) { return g_repairs; }
+
+/* Residual, forward-Euler state W = Y + dt L and the stage output of one cell. */
+static int cell_update_(const rpfvm_config *cfg, const double *Un, const double *Y, double *out,
+                        const work_t *w, int kind, double dt, int i, int j) {
+    const dims_t d = dims_of(cfg);
+    const double idx = 1.0 / cfg->dx, idy = 1.0 / cfg->dy;
+    double W[4];
+    for (int k = 0; k < 4; k++) {
+        double l = -(AT(w->F, k, i, j) - AT(w->F, k, i - 1, j)) * idx
+                   - (AT(w->G, k, i, j) - AT(w->G, k, i, j - 1)) * idy + AT(w->S, k, i, j);
+        AT(w->L, k, i, j) = l;
+        W[k] = AT(Y, k, i, j) + dt * l;
+        AT(out, k, i, j) = combine_(kind, AT(Un, k, i, j), AT(Y, k, i, j), l, dt);
+    }
+    return admissible_(cfg->gamma, W);
+}
+
+/* Positivity safeguard, an extension the reference does not have (a posteriori, cell
+ * based): a cell whose forward-Euler state W = Y + dt L has rho <= 0 or p <= 0 gets all
+ * four of its face fluxes replaced by the first-order HLL flux of the adjacent cells,
+ * and every cell touching a replaced face is updated again; cells that become
+ * inadmissible are repaired in the next round (at most 8 rounds).  Every step here is
+ * order-independent, so the device path reproduces it exactly. */
+static int repair_(const rpfvm_config *cfg, const double *Un, const double *Y, double *out,
+                   work_t *w, int kind, double dt, unsigned char *state) {
+    const dims_t d = dims_of(cfg);
+    const int nx = cfg->nx, ny = cfg->ny;
+    const size_t ncell = (size_t)nx * ny;
+    /* state bit 1: in current list, bit 2: repaired, bit 4: touched this round */
+    int *list = malloc(sizeof(int) * ncell), *next = malloc(sizeof(int) * ncell);
+    int *touched = malloc(sizeof(int) * 5 * ncell);
+    int nlist = 0, faults = 0;
+    for (int i = GC; i < d.R - GC; i++)
+        for (int j = GC; j < d.C - GC; j++) {
+            double W[4];
+            for (int k = 0; k < 4; k++) W[k] = AT(Y, k, i, j) + dt * AT(w->L, k, i, j);
+            if (!admissible_(cfg->gamma, W)) list[nlist++] = (i - GC) * ny + (j - GC);
+        }
+    int rounds = 0;
+    while (nlist > 0 && rounds < 8) {
+        for (int q = 0; q < nlist; q++) {
+            const int i = GC + list[q] / ny, j = GC + list[q] % ny;
+            state[list[q]] |= 2;
+            double a[4], b[4], f[4];
+            for (int side = 0; side < 4; side++) {
+                /* faces [i-1][j], [i][j] (x) and [i][j-1], [i][j] (y) */
+                const int dir = side < 2 ? 1 : 2;
+                const int fi = side == 0 ? i - 1 : i, fj = side == 2 ? j - 1 : j;
+                for (int k = 0; k < 4; k++) {
+                    a[k] = AT(Y, k, fi, fj);
+                    b[k] = dir == 1 ? AT(Y, k, fi + 1, fj) : AT(Y, k, fi, fj + 1);
+                }
+                faults |= hlle_(cfg->gamma, dir, a, b, f);
+                for (int k = 0; k < 4; k++) {
+                    if (dir == 1) AT(w->F, k, fi, fj) = f[k];
+                    else AT(w->G, k, fi, fj) = f[k];
+                }
+            }
+        }
+        int nt = 0;
+        for (int q = 0; q < nlist; q++) {
+            const int ci = list[q] / ny, cj = list[q] % ny;
+            const int di[5] = {0, -1, 1, 0, 0}, dj[5] = {0, 0, 0, -1, 1};
+            for (int m = 0; m < 5; m++) {
+                const int ti = ci + di[m], tj = cj + dj[m];
+                if (ti < 0 || ti >= nx || tj < 0 || tj >= ny) continue;
+                const int c = ti * ny + tj;
+                if (!(state[c] & 4)) {
+                    state[c] |= 4;
+                    touched[nt++] = c;
+                }
+            }
+        }
+        int nnext = 0;
+        for (int q = 0; q < nt; q++) {
+            const int c = touched[q];
+            state[c] &= ~4;
+            const int ok = cell_update_(cfg, Un, Y, out, w, kind, dt, GC + c / ny, GC + c % ny);
+            if (!ok && !(state[c] & 2)) next[nnext++] = c;
+            else if (!ok) faults |= 8;
+        }
+        g_repairs += nlist;
+        int *t = list;
+        list = next;
+        next = t;
+        nlist = nnext;
+        rounds++;
+    }
+    if (nlist > 0) faults |= 8;
+    free(list);
+    free(next);
+    free(touched);
+    return faults ? (faults & 1 ? RPFVM_FAULT_HLL_BAD_STATE : 0) | (faults & 8 ? RPFVM_FAULT_POSITIVITY : 0) : 0;
+}
+
+/* One RK stage: out = combine(kind, Un, Y, dt L(Y)).  out may alias Un. */
+static int stage_(const rpfvm_config *cfg, const double *Un, double *Y, double *out, int kind,
+                  double dt, work_t *w, double *tmp, unsigned char *state) {
+    const dims_t d = dims_of(cfg);
+    int bad = stage_operator_(cfg, Y, w);
+    double *dst = cfg->positivity ? tmp : out;
+#pragma omp parallel for collapse(2)
+    for (int i = GC; i <= d.R - GC - 1; i++)
+        for (int j = GC; j <= d.C - GC - 1; j++)
+            for (int k = 0; k < 4; k++)
+                AT(dst, k, i, j) = combine_(kind, AT(Un, k, i, j), AT(Y, k, i, j), AT(w->L, k, i, j), dt);
+    if (cfg->positivity) {
+        memset(state, 0, (size_t)cfg->nx * cfg->ny);
+        bad |= repair_(cfg, Un, Y, dst, w, kind, dt, state);
+        for (int k = 0; k < 4; k++)
+            for (int i = GC; i <= d.R - GC - 1; i++)
+                memcpy(&AT(out, k, i, GC), &AT(dst, k, i, GC), sizeof(double) * cfg->ny);
+    }
+    return bad;
+}
+
 int orc_time_advance(const rpfvm_config *cfg, double *U, double dt) {
     int e = check_cfg_(cfg);
     if (e) return e;
-    const dims_t d = dims_of(cfg);
     work_t w;
     if (work_alloc_(cfg, &w)) return RPFVM_ERR_NOMEM;
     double *U1 = calloc(w.n, sizeof(double)), *U2 = calloc(w.n, sizeof(double));
-    if (!U1 || !U2) {
-        free(U1); free(U2); work_free_(&w);
+    double *tmp = cfg->positivity ? calloc(w.n, sizeof(double)) : NULL;
+    unsigned char *state = cfg->positivity ? calloc((size_t)cfg->nx * cfg->ny, 1) : NULL;
+    if (!U1 || !U2 || (cfg->positivity && (!tmp || !state))) {
+        free(U1); free(U2); free(tmp); free(state); work_free_(&w);
         return RPFVM_ERR_NOMEM;
     }
-    double *L = w.L;
     int bad = 0;
+    g_repairs = 0;
     /* RK1_TimeAd :643-708, RK2_TimeAd :711-785, RK3_TimeAd :788-882 */
-    bad |= stage_operator_(cfg, U, &w);
     if (cfg->time_order == 1) {
-        for (int k = 0; k < 4; k++)
-            for (int i = GC; i <= d.R - GC - 1; i++)
-                for (int j = GC; j <= d.C - GC - 1; j++)
-                    AT(U, k, i, j) = AT(U, k, i, j) + dt * AT(L, k, i, j);
+        bad |= stage_(cfg, U, U, U, K_EULER, dt, &w, tmp, state);
+    } else if (cfg->time_order == 2) {
+        bad |= stage_(cfg, U, U, U1, K_EULER, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U1, U, K_RK2_2, dt, &w, tmp, state);
     } else {
-        for (int k = 0; k < 4; k++)
-            for (int i = GC; i <= d.R - GC - 1; i++)
-                for (int j = GC; j <= d.C - GC - 1; j++)
-                    AT(U1, k, i, j) = AT(U, k, i, j) + dt * AT(L, k, i, j);
-        bad |= stage_operator_(cfg, U1, &w);
-        if (cfg->time_order == 2) {
-            for (int k = 0; k < 4; k++)
-                for (int i = GC; i <= d.R - GC - 1; i++)
-                    for (int j = GC; j <= d.C - GC - 1; j++)
-                        AT(U, k, i, j) = 0.5 * AT(U, k, i, j)
-                                       + 0.5 * (AT(U1, k, i, j) + dt * AT(L, k, i, j));
-        } else {
-            for (int k = 0; k < 4; k++)
-                for (int i = GC; i <= d.R - GC - 1; i++)
-                    for (int j = GC; j <= d.C - GC - 1; j++)
-                        AT(U2, k, i, j) = (3.0 / 4.0) * AT(U, k, i, j)
-                                        + (1.0 / 4.0) * (AT(U1, k, i, j) + dt * AT(L, k, i, j));
-            bad |= stage_operator_(cfg, U2, &w);
-            for (int k = 0; k < 4; k++)
-                for (int i = GC; i <= d.R - GC - 1; i++)
-                    for (int j = GC; j <= d.C - GC - 1; j++)
-                        AT(U, k, i, j) = (1.0 / 3.0) * AT(U, k, i, j)
-                                       + (2.0 / 3.0) * (AT(U2, k, i, j) + dt * AT(L, k, i, j));
-        }
+        bad |= stage_(cfg, U, U, U1, K_EULER, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U1, U2, K_RK3_2, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U2, U, K_RK3_3, dt, &w, tmp, state);
     }
     free(U1);
     free(U2);
+    free(tmp);
+    free(state);
     work_free_(&w);
     return bad;
 }

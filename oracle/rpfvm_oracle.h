@@ -51,6 +51,9 @@ void orc_weno5_points(int n, const double *in6, double *out4);
  * Compute_Eigen_2DX/2DY); EL/ER are [4][4][R][C] (zero where no face). */
 int orc_eigen(const rpfvm_config *cfg, const double *U, int dir, double *EL, double *ER);
 
+/* Cells repaired by the positivity safeguard during the last orc_time_advance. */
+long orc_last_repairs(void);
+
 /* Number of OpenMP threads the oracle runs with. */
 int orc_threads(void);
 

@@ -51,6 +51,9 @@ struct rpfvm_ctx {
     BcSide bc[4]{};
     void* pool = nullptr;
     double *U = nullptr, *U1 = nullptr, *U2 = nullptr, *F = nullptr, *G = nullptr, *L = nullptr;
+    double* U3 = nullptr;          // stage output when the positivity safeguard is on
+    Repair rp{};                   // safeguard lists (null pointers when off)
+    unsigned stamp = 0;            // stage stamp of the safeguard
     double* partial = nullptr;
     double* scal = nullptr;        // [0] dt, [1] t, [2] t_stop
     unsigned* faults = nullptr;
@@ -190,19 +193,36 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
 
     const size_t arr = 4 * x->g.plane + 32;             // doubles per state-sized array
     const size_t arr_bytes = ((arr * sizeof(double) + 255) / 256) * 256;
+    const int narr = c.positivity ? 7 : 6;
     const int nb = dt_blocks(x->g);
     const size_t small = 256 * 4 + (size_t)nb * 2 * sizeof(double) + 256;
-    he = hipMalloc(&x->pool, 6 * arr_bytes + small);
+    const size_t ncell = (size_t)c.nx * c.ny;
+    const size_t cell_bytes = ((ncell * 4 + 255) / 256) * 256;
+    const size_t rep_bytes = c.positivity ? 5 * cell_bytes + 256 : 0;
+    const size_t total = narr * arr_bytes + small + rep_bytes;
+    he = hipMalloc(&x->pool, total);
     if (he != hipSuccess)
-        return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", 6 * arr_bytes + small, hipGetErrorString(he)));
+        return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(he)));
     char* p = static_cast<char*>(x->pool);
-    double** arrays[6] = {&x->U, &x->U1, &x->U2, &x->F, &x->G, &x->L};
-    for (int k = 0; k < 6; k++) *arrays[k] = reinterpret_cast<double*>(p + k * arr_bytes) + kFrontPad;
-    char* q = p + 6 * arr_bytes;
+    double** arrays[7] = {&x->U, &x->U1, &x->U2, &x->F, &x->G, &x->L, &x->U3};
+    for (int k = 0; k < narr; k++) *arrays[k] = reinterpret_cast<double*>(p + k * arr_bytes) + kFrontPad;
+    char* q = p + narr * arr_bytes;
     x->scal = reinterpret_cast<double*>(q);
     x->faults = reinterpret_cast<unsigned*>(q + 256);
     x->partial = reinterpret_cast<double*>(q + 512);
-    he = hipMemsetAsync(x->pool, 0, 6 * arr_bytes + small, x->stream);
+    if (c.positivity) {
+        char* r = q + small;
+        x->rp.cnt = reinterpret_cast<unsigned*>(r);
+        r += 256;
+        x->rp.list[0] = reinterpret_cast<int*>(r);
+        x->rp.list[1] = reinterpret_cast<int*>(r + cell_bytes);
+        x->rp.touched = reinterpret_cast<int*>(r + 2 * cell_bytes);
+        x->rp.rep = reinterpret_cast<unsigned*>(r + 3 * cell_bytes);
+        x->rp.tch = reinterpret_cast<unsigned*>(r + 4 * cell_bytes);
+        x->rp.nx = c.nx;
+        x->rp.ny = c.ny;
+    }
+    he = hipMemsetAsync(x->pool, 0, total, x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
     he = hipHostMalloc(reinterpret_cast<void**>(&x->host_scal), 256, hipHostMallocDefault);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(he)));
@@ -271,9 +291,13 @@ static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
 }
 
 static void enqueue_update(rpfvm_ctx* x, const double* Un, const double* Y, double* out,
-                           double dt, int kind) {
+                           double dt, int kind, bool repair) {
     Timed t(x, RPFVM_KCLASS_UPDATE);
-    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->stream);
+    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, repair ? x->rp : Repair{},
+                  x->stream);
+    if (repair && x->rp.cnt)
+        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, ++x->stamp, x->faults,
+                      x->stream);
 }
 
 static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
@@ -286,23 +310,32 @@ static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
                           int kind) {
     enqueue_bc(x, Y);
     enqueue_fluxes(x, Y);
-    enqueue_update(x, Un, Y, out, dt, kind);
+    enqueue_update(x, Un, Y, out, dt, kind, true);
 }
 
 static int enqueue_step(rpfvm_ctx* x, double dt) {
+    // With the positivity safeguard the last stage cannot update U in place (a repaired
+    // neighbour is recomputed from U^n): it writes U3, which takes over U's ghost frame
+    // (the reference leaves U's ghosts as filled at stage 1) and becomes the state.
+    double* last = x->rp.cnt ? x->U3 : x->U;
     switch (x->cfg.time_order) {
         case 1:
-            enqueue_stage(x, x->U, x->U, x->U, dt, UPD_EULER);
+            enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER);
             break;
         case 2:
             enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
-            enqueue_stage(x, x->U, x->U1, x->U, dt, UPD_RK2_2);
+            enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2);
             break;
         default:
             enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
             enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2);
-            enqueue_stage(x, x->U, x->U2, x->U, dt, UPD_RK3_3);
+            enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3);
             break;
+    }
+    if (last != x->U) {
+        launch_copy_frame(x->U, last, x->g, x->stream);
+        x->U3 = x->U;
+        x->U = last;
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -334,7 +367,7 @@ int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
     enqueue_bc(x, x->U);
     enqueue_fluxes(x, x->U);
-    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL);
+    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, false);
     HIPCHK(hipGetLastError());
     int e = 0;
     if (L && (e = copy_out(x, L, x->L))) return e;

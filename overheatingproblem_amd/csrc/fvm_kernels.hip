@@ -97,15 +97,14 @@ __global__ __launch_bounds__(256) void k_flux(const double* __restrict__ Y, doub
 
 // ------------------------------------------------------------------ update
 // Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
-// fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872).
-// `out` may alias `Un` (last stage writes the state in place): each thread reads
-// its own cell before writing it.
-__global__ __launch_bounds__(256) void k_update(const double* Un, const double* Y, double* out, const double* __restrict__ F,
-                                                const double* __restrict__ G, Geom g, Phys ph,
-                                                double dt, int kind) {
-    const int j = GC + blockIdx.x * 64 + threadIdx.x;
-    const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= g.R - GC || j >= g.C - GC) return;
+// fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872) for
+// cell (i, j).  `out` may alias `Un` (the last stage writes the state in place): each
+// thread reads its own cell before writing it.  Returns whether the forward-Euler
+// state W = Y + dt L has rho > 0 and p > 0 (used by the positivity safeguard).
+__device__ __forceinline__ bool cell_update(const double* Un, const double* Y, double* out,
+                                            const double* __restrict__ F,
+                                            const double* __restrict__ G, const Geom& g,
+                                            const Phys& ph, double dt, int kind, int i, int j) {
     const size_t c = (size_t)i * g.ld + j;
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
@@ -115,20 +114,151 @@ __global__ __launch_bounds__(256) void k_update(const double* Un, const double* 
         S[3] = rho * ph.gravity * v;
     }
     const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+    double W[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const size_t o = (size_t)k * g.plane + c;
         const double L = -(F[o] - F[o - g.ld]) * idx - (G[o] - G[o - 1]) * idy + S[k];
+        const double y = Y[o];
+        W[k] = y + dt * L;
         double r;
         switch (kind) {
             case UPD_RESIDUAL: r = L; break;                                   // L only
-            case UPD_EULER: r = Y[o] + dt * L; break;                          // U1 = Y + dt L
-            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (Y[o] + dt * L); break;
-            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (Y[o] + dt * L); break;
-            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (Y[o] + dt * L); break;  // UPD_RK3_3
+            case UPD_EULER: r = y + dt * L; break;                             // U1 = Y + dt L
+            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (y + dt * L); break;
+            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (y + dt * L); break;
+            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (y + dt * L); break;  // UPD_RK3_3
         }
         out[o] = r;
     }
+    return admissible(ph.gamma, W);
+}
+
+__global__ __launch_bounds__(256) void k_update(const double* Un, const double* Y, double* out,
+                                                const double* __restrict__ F,
+                                                const double* __restrict__ G, Geom g, Phys ph,
+                                                double dt, int kind, Repair rp) {
+    const int j = GC + blockIdx.x * 64 + threadIdx.x;
+    const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.R - GC || j >= g.C - GC) return;
+    const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+    if (rp.cnt && !ok) {
+        const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+        rp.list[0][k] = (i - GC) * rp.ny + (j - GC);
+    }
+}
+
+// ------------------------------------------------------ positivity safeguard
+// An extension the reference does not have.  A cell whose forward-Euler state is
+// inadmissible gets its four face fluxes replaced by the first-order HLLE flux of the
+// adjacent cells; every cell touching a replaced face is updated again, and cells that
+// become inadmissible are repaired in the next round.  Each kernel works on a list
+// (grid-stride), so a round with an empty list costs one small launch.
+__global__ __launch_bounds__(256) void k_repair_faces(const double* __restrict__ Y, double* F,
+                                                      double* G, Geom g, Phys ph, Repair rp,
+                                                      int round, unsigned stamp,
+                                                      unsigned* __restrict__ faults) {
+    const int cur = round & 1;
+    const unsigned n = rp.cnt[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rp.cnt[cur ^ 1] = 0;
+        rp.cnt[2] = 0;
+        if (n) atomicAdd(&rp.cnt[3], n);
+    }
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int cell = rp.list[cur][q];
+        rp.rep[cell] = stamp;
+        const int i = GC + cell / rp.ny, j = GC + cell % rp.ny;
+#pragma unroll
+        for (int side = 0; side < 4; side++) {
+            const int fi = side == 0 ? i - 1 : i, fj = side == 2 ? j - 1 : j;
+            const size_t c = (size_t)fi * g.ld + fj;
+            const size_t nb = side < 2 ? c + g.ld : c + 1;
+            double a[4], b[4], f[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                a[k] = Y[(size_t)k * g.plane + c];
+                b[k] = Y[(size_t)k * g.plane + nb];
+            }
+            const int bad = side < 2 ? hlle<1>(ph.gamma, a, b, f) : hlle<2>(ph.gamma, a, b, f);
+            if (bad) atomicOr(faults, 1u);
+            double* D = side < 2 ? F : G;
+#pragma unroll
+            for (int k = 0; k < 4; k++) D[(size_t)k * g.plane + c] = f[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_repair_touch(Repair rp, int round, unsigned tstamp) {
+    const int cur = round & 1;
+    const unsigned n = rp.cnt[cur];
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 5 * n; q += gridDim.x * blockDim.x) {
+        const int cell = rp.list[cur][q / 5];
+        const int m = q % 5;
+        int ti = cell / rp.ny, tj = cell % rp.ny;
+        ti += m == 1 ? -1 : m == 2 ? 1 : 0;
+        tj += m == 3 ? -1 : m == 4 ? 1 : 0;
+        if (ti < 0 || ti >= rp.nx || tj < 0 || tj >= rp.ny) continue;
+        const int c = ti * rp.ny + tj;
+        if (atomicExch(&rp.tch[c], tstamp) != tstamp) {
+            const unsigned k = atomicAdd(&rp.cnt[2], 1u);
+            rp.touched[k] = c;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_repair_cells(const double* Un, const double* Y, double* out,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ G, Geom g,
+                                                      Phys ph, double dt, int kind, Repair rp,
+                                                      int round, unsigned stamp,
+                                                      unsigned* __restrict__ faults) {
+    const int nxt = (round & 1) ^ 1;
+    const unsigned n = rp.cnt[2];
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int c = rp.touched[q];
+        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, GC + c / rp.ny,
+                                    GC + c % rp.ny);
+        if (!ok) {
+            if (rp.rep[c] != stamp) {
+                const unsigned k = atomicAdd(&rp.cnt[nxt], 1u);
+                rp.list[nxt][k] = c;
+            } else {
+                atomicOr(faults, 8u);
+            }
+        }
+    }
+}
+
+// After the last round: cells still listed were never repaired; reset the counters.
+__global__ void k_repair_finish(Repair rp, int last_list, unsigned* __restrict__ faults) {
+    if (threadIdx.x == 0) {
+        if (rp.cnt[last_list]) atomicOr(faults, 8u);
+        rp.cnt[0] = 0;
+        rp.cnt[1] = 0;
+        rp.cnt[2] = 0;
+    }
+}
+
+// Copies the ghost frame of src into dst (all four variables).
+__global__ void k_copy_frame(const double* __restrict__ src, double* __restrict__ dst, Geom g) {
+    const int nframe = 2 * GC * g.C + 2 * GC * (g.R - 2 * GC);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nframe) return;
+    int i, j;
+    if (t < 2 * GC * g.C) {
+        const int r = t / g.C;
+        i = r < GC ? r : g.R - 2 * GC + r;
+        j = t % g.C;
+    } else {
+        const int u = t - 2 * GC * g.C;
+        i = GC + u / (2 * GC);
+        const int k = u % (2 * GC);
+        j = k < GC ? k : g.C - 2 * GC + k;
+    }
+    const size_t c = (size_t)i * g.ld + j;
+#pragma unroll
+    for (int v = 0; v < 4; v++) dst[(size_t)v * g.plane + c] = src[(size_t)v * g.plane + c];
 }
 
 // ------------------------------------------------------------------- CFL dt
@@ -257,11 +387,29 @@ void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st) {
 
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
                    const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                   hipStream_t s) {
+                   const Repair& rp, hipStream_t s) {
     const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
-    hipLaunchKernelGGL(k_update, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind);
+    hipLaunchKernelGGL(k_update, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp);
+}
+
+void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
+                   const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                   unsigned stamp, unsigned* faults, hipStream_t s) {
+    const dim3 block(256), grid(64);
+    for (int r = 0; r < kRepairRounds; r++) {
+        hipLaunchKernelGGL(k_repair_faces, grid, block, 0, s, Y, F, G, g, ph, rp, r, stamp, faults);
+        hipLaunchKernelGGL(k_repair_touch, grid, block, 0, s, rp, r, stamp * 16u + (unsigned)r);
+        hipLaunchKernelGGL(k_repair_cells, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp,
+                           r, stamp, faults);
+    }
+    hipLaunchKernelGGL(k_repair_finish, dim3(1), dim3(64), 0, s, rp, kRepairRounds & 1, faults);
+}
+
+void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s) {
+    const int nframe = 2 * GC * g.C + 2 * GC * (g.R - 2 * GC);
+    hipLaunchKernelGGL(k_copy_frame, dim3((nframe + 255) / 256), dim3(256), 0, s, src, dst, g);
 }
 
 int dt_blocks(const Geom& g) {

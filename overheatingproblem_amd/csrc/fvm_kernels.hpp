@@ -33,6 +33,18 @@ struct BcSide {
 
 enum : int { UPD_RESIDUAL = 0, UPD_EULER = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
 
+// Device lists of the positivity safeguard (all null when it is off).  Cells are
+// numbered over the interior, c = (i - GC) * ny + (j - GC).
+struct Repair {
+    unsigned* cnt;       // [0], [1]: list lengths, [2]: touched length, [3]: repairs total
+    int* list[2];        // cells to repair in the current / next round
+    int* touched;        // cells to update again in the current round
+    unsigned* rep;       // rep[c] == stage stamp: c was repaired in this stage
+    unsigned* tch;       // tch[c] == round stamp: c is already in the touched list
+    int nx, ny;
+};
+constexpr int kRepairRounds = 8;
+
 using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phys& ph,
                             unsigned* faults, hipStream_t s);
 struct FluxPair {
@@ -43,7 +55,11 @@ FluxPair select_flux(int order, bool characteristic, int solver);
 void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st);
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
                    const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                   hipStream_t s);
+                   const Repair& rp, hipStream_t s);
+void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
+                   const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                   unsigned stamp, unsigned* faults, hipStream_t s);
+void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s);
 int dt_blocks(const Geom& g);
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,
                const double* clock, unsigned* faults, hipStream_t s);

@@ -432,6 +432,50 @@ __device__ __forceinline__ int hll(double g, const double cl[4], const double cr
     return 0;
 }
 
+// Positively conservative HLLE (Einfeldt, Munz, Roe & Sjogreen 1991): wave-speed
+// bounds s_L = min(q_L - a_L, q~ - c~), s_R = max(q_R + a_R, q~ + c~) with Roe
+// averages.  Used only by the positivity safeguard (an extension): the reference's HLL
+// uses the unbounded speeds q~ -/+ d, which do not preserve positivity.
+template <int DIR>
+__device__ __forceinline__ int hlle(double g, const double cl[4], const double cr[4], double F[4]) {
+    if (!(cl[0] > 0.0) || !(cr[0] > 0.0)) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
+        return 1;
+    }
+    const Side L = make_side(g, cl), R = make_side(g, cr);
+    const double gr = g / (g - 1.0);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    const double aL = sqrt(fmax(g * L.p * L.ir, 0.0));
+    const double aR = sqrt(fmax(g * R.p * R.ir, 0.0));
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double iss = 1.0 / (sL + sR);
+    const double ubar = (sL * L.u + sR * R.u) * iss;
+    const double vbar = (sL * L.v + sR * R.v) * iss;
+    const double Hbar = (sL * HL + sR * HR) * iss;
+    const double cbar = sqrt(fmax((g - 1.0) * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)), 0.0));
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const double qbar = DIR == 1 ? ubar : vbar;
+    const double sl = fmin(qL - aL, qbar - cbar);
+    const double sr = fmax(qR + aR, qbar + cbar);
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sr <= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    } else {
+        const double isd = 1.0 / (sr - sl);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = (sr * FL[k] - sl * FR[k] + sl * sr * (cr[k] - cl[k])) * isd;
+    }
+    return 0;
+}
+
 // Roe with the reference's no-op entropy fix, include/scheme.h:1018-1322.
 template <int DIR>
 __device__ __forceinline__ int roe(double g, const double cl[4], const double cr[4], double F[4]) {

@@ -130,16 +130,34 @@ def test_oracle_boundaries(oracle, bc):
     assert same(G, U), first_diff(G, U)
 
 
-def test_oracle_noh_positivity(oracle):
-    """Noh (p = 1e-6) with the face-level positivity fallback switched on."""
-    c, U0 = cases.noh(96, positivity=True)
+@pytest.mark.parametrize("case", ["noh64", "implosion48"])
+def test_oracle_positivity_safeguard(oracle, case):
+    """Positivity safeguard (face fallback + cell repair) vs the oracle, bit-exact, on
+    runs long enough for repairs to fire (Noh p = 1e-6; a periodic implosion whose
+    walls pull toward vacuum)."""
+    from test_oracle_positivity import periodic_implosion
+    if case == "noh64":
+        c, U0 = cases.noh(64, positivity=True)
+        steps = 40
+    else:
+        c, U0 = periodic_implosion(48)
+        steps = 60
     U = U0.copy()
-    ref_dts = oracle.run_steps(c, U, 6)
+    ref_dts, repairs = [], 0
+    for _ in range(steps):
+        ref_dts.append(oracle.delta_t(c, U))
+        assert oracle.time_advance(c, U, ref_dts[-1]) == 0
+        repairs += oracle.last_repairs()
+    assert repairs > 0
     with Solver(c) as s:
         s.set_state(U0)
-        for dt in ref_dts:
-            s.time_advance(dt)
+        dts = []
+        for _ in range(steps):
+            dts.append(s.get_delta_t())
+            s.time_advance(dts[-1])
         G = s.get_state()
+        assert s.sync() == 0
+    assert same(np.array(dts), np.array(ref_dts))
     assert same(G, U), first_diff(G, U)
 
 
