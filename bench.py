@@ -80,13 +80,22 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def run_gpu(cfg, U0, steps, warmup):
+def run_gpu(cfg, U0, steps, warmup, dts=None):
+    """Warm-up + timed steps with dt from the device reduction (or replay `dts`, untimed
+    use).  Returns timings, kernel times, fault bits, final state and the dts used."""
     from overheatingproblem_amd.capi import Solver
     ev = HipEvents()
+    used = []
     with Solver(cfg) as s:
         s.set_state(U0)
-        for _ in range(warmup):
-            s.time_advance(s.get_delta_t())
+
+        def step(k):
+            dt = dts[k] if dts is not None else s.get_delta_t()
+            used.append(dt)
+            s.time_advance(dt)
+
+        for k in range(warmup):
+            step(k)
         faults = s.sync()
         stream = s.stream()
         s.set_kernel_timing(True)
@@ -94,15 +103,16 @@ def run_gpu(cfg, U0, steps, warmup):
         e0, e1 = ev.create(), ev.create()
         ev.record(e0, stream)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            s.time_advance(s.get_delta_t())
+        for k in range(warmup, warmup + steps):
+            step(k)
         ev.record(e1, stream)
         gpu_ms = ev.elapsed_ms(e0, e1)
         t1 = time.perf_counter()
         kt = s.kernel_times()
         faults |= s.sync()
         U = s.get_state()
-    return {"gpu_ms": gpu_ms, "wall_s": t1 - t0, "kernels": kt, "faults": faults, "U": U}
+    return {"gpu_ms": gpu_ms, "wall_s": t1 - t0, "kernels": kt, "faults": faults, "U": U,
+            "dts": used}
 
 
 def cpu_baseline(n, steps, cores):
@@ -151,16 +161,32 @@ def main():
         sys.exit(2)
 
     from overheatingproblem_amd import cases
+    from overheatingproblem_amd.config import PRECISION_MIXED
     n = args.n
     cfg, U0 = cases.noh(n, positivity=True)
+    cfg_mx = cfg.copy()
+    cfg_mx.precision = PRECISION_MIXED
     cells = n * n
-    log(f"bench: noh {n}x{n} fp64 warmup={args.warmup} steps={args.steps}")
-    r = run_gpu(cfg, U0, args.steps, args.warmup)
-    ms_per_step = r["gpu_ms"] / args.steps
-    value = cells * args.steps / (r["gpu_ms"] / 1e3) / 1e6
-    log(f"bench: {ms_per_step:.3f} ms/step, {value:.1f} Mcell-updates/s, faults={r['faults']}")
+    log(f"bench: noh {n}x{n} warmup={args.warmup} steps={args.steps}: fp64 ...")
+    r64 = run_gpu(cfg, U0, args.steps, args.warmup)
+    log(f"bench: fp64 {r64['gpu_ms'] / args.steps:.3f} ms/step; mixed ...")
+    rmx = run_gpu(cfg_mx, U0, args.steps, args.warmup)
+    log(f"bench: mixed {rmx['gpu_ms'] / args.steps:.3f} ms/step; mixed replay on fp64 dts ...")
+    rep = run_gpu(cfg_mx, U0, args.steps, args.warmup, dts=r64["dts"])
 
-    kt = r["kernels"]
+    def rate(r):
+        return cells * args.steps / (r["gpu_ms"] / 1e3) / 1e6
+
+    value, value64 = rate(rmx), rate(r64)
+    ms_per_step = rmx["gpu_ms"] / args.steps
+    rho64 = r64["U"][0, 4:-4, 4:-4]
+    rhomx = rep["U"][0, 4:-4, 4:-4]
+    l1 = float(np.abs(rhomx - rho64).mean())
+    linf = float(np.abs(rhomx - rho64).max())
+    log(f"bench: mixed {value:.1f} vs fp64 {value64:.1f} Mcell-updates/s "
+        f"(x{value / value64:.2f}), L1(rho) {l1:.3e}")
+
+    kt = rmx["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_n = kt[dom]
     avg_ms = dom_ms / max(1, dom_n)
@@ -172,7 +198,12 @@ def main():
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     step_gbs = STEP_BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
 
-    rho = r["U"][0, 4:-4, 4:-4]
+    def form(r, v):
+        return {"value": round(v, 3), "ms_per_step": round(r["gpu_ms"] / args.steps, 4),
+                "step_hbm_frac": round(STEP_BYTES_PER_CELL * cells * v * 1e6 / cells / 8e12, 5),
+                "kernel_ms_per_step": {k: round(t[0] / args.steps, 4) for k, t in r["kernels"].items()},
+                "faults": r["faults"], "finite": bool(np.isfinite(r["U"]).all())}
+
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -184,22 +215,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"noh-{n}x{n}", "scheme": "WENO5-char(Z) + HLLC, SSP-RK3",
-                   "cfl": 0.4, "gamma": "5/3", "p0": 1e-6, "form": "fp64",
-                   "positivity_fallback": True, "cells": cells},
-        "roofline": {"bound": "hbm", "kernel": f"k_flux<{dom}>" if dom.startswith("flux") else dom,
+                   "cfl": 0.4, "gamma": "5/3", "p0": 1e-6, "form": "mixed",
+                   "positivity_safeguard": True, "cells": cells},
+        "roofline": {"bound": "hbm", "kernel": f"k_flux_mixed ({dom})" if dom.startswith("flux") else dom,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "avg_launch_ms": round(avg_ms, 4), "launches": dom_n,
                      "alg_bytes_per_launch": alg_bytes},
         "step_hbm": {"achieved_gbs": round(step_gbs, 2), "frac": round(step_gbs / HBM_PEAK_GBS, 5),
                      "bytes_per_cell_update": STEP_BYTES_PER_CELL},
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
-        "wall_s": round(r["wall_s"], 4),
-        "faults": r["faults"],
-        "finite": bool(np.isfinite(rho).all()),
+        "forms": {"mixed": form(rmx, value), "fp64": form(r64, value64),
+                  "speedup_mixed_vs_fp64": round(value / value64, 4)},
+        "accuracy": {"l1_rho_mixed_vs_fp64": l1, "linf_rho_mixed_vs_fp64": linf,
+                     "note": f"mixed replayed on the fp64 run's {len(r64['dts'])} dts"},
+        "wall_s": round(rmx["wall_s"], 4),
     }
     if not args.no_cpu:
         log(f"bench: cpu baseline ({args.cpu_steps} steps at {n}^2) ...")

@@ -61,6 +61,9 @@ enum {
                                         or p <= 0 */
 };
 
+/* rpfvm_config.debug_flags bits. */
+enum { RPFVM_DEBUG_MIXED_ALL_FP64 = 1 };
+
 /* Error codes (< 0). */
 enum {
     RPFVM_OK = 0,
@@ -91,7 +94,9 @@ typedef struct rpfvm_config {
                                   rho or p is not positive (or not finite) falls back to
                                   first-order (piecewise-constant) states.  Not in the
                                   reference, which has no positivity safeguard. */
-    int32_t reserved0;         /* must be 0 */
+    int32_t debug_flags;       /* 0 normally.  RPFVM_DEBUG_MIXED_ALL_FP64: the mixed form
+                                  routes every face through its FP64 recompute list
+                                  (verification of that path only) */
     int32_t bc[4];             /* left, right, bottom, top: RPFVM_BC_* */
     double dx, dy;
     double gamma, cfl, gravity;

@@ -29,7 +29,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract
           "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable"]
 
-SOURCES = ["fvm_kernels.hip", "fvm_api.cpp"]
+SOURCES = ["fvm_kernels.hip", "fvm_mixed_kernels.hip", "fvm_api.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -20,7 +20,8 @@ SCHEME_NAMES = {1: "HLL", 2: "HLLC", 3: "Roe", 11: "HLLHC", 22: "HLLCHC", 33: "R
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
-FAULT_HLL_BAD_STATE, FAULT_NONFINITE_DT, FAULT_MIXED_FALLBACK = 1, 2, 4
+FAULT_HLL_BAD_STATE, FAULT_NONFINITE_DT, FAULT_MIXED_FALLBACK, FAULT_POSITIVITY = 1, 2, 4, 8
+DEBUG_MIXED_ALL_FP64 = 1
 
 # The reference's default inflow / fixed-value states: a Mach-3 stream of
 # rho = 1.4, u = 3, p = 1 in conserved form (include/Boundary_Condition.h:49,
@@ -41,7 +42,7 @@ class Config(ctypes.Structure):
         ("source", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("positivity", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("debug_flags", ctypes.c_int32),
         ("bc", ctypes.c_int32 * 4),
         ("dx", ctypes.c_double),
         ("dy", ctypes.c_double),

@@ -53,6 +53,7 @@ struct rpfvm_ctx {
     double *U = nullptr, *U1 = nullptr, *U2 = nullptr, *F = nullptr, *G = nullptr, *L = nullptr;
     double* U3 = nullptr;          // stage output when the positivity safeguard is on
     Repair rp{};                   // safeguard lists (null pointers when off)
+    FaceFix fix{};                 // mixed form: faces recomputed in FP64
     unsigned stamp = 0;            // stage stamp of the safeguard
     double* partial = nullptr;
     double* scal = nullptr;        // [0] dt, [1] t, [2] t_stop
@@ -140,8 +141,8 @@ static int validate(const rpfvm_config* c) {
         return fail(RPFVM_ERR_INVALID, "recon_order must be 1, 2, 3 or 5");
     if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
         return fail(RPFVM_ERR_UNSUPPORTED, "scheme %d is not implemented on the device", c->scheme);
-    if (c->precision != RPFVM_PRECISION_FP64)
-        return fail(RPFVM_ERR_UNSUPPORTED, "precision %d is not implemented", c->precision);
+    if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
+        return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 4) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
     if (!(c->dx > 0) || !(c->dy > 0) || !(c->gamma > 1.0))
@@ -167,9 +168,15 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ph.dy = c.dy;
     x->ph.cfl = c.cfl;
     x->ph.gravity = c.gravity;
+    x->ph.gm1 = c.gamma - 1.0;
+    x->ph.gr = c.gamma / (c.gamma - 1.0);
+    x->ph.six = 6.0;
     x->ph.source = c.source ? 1 : 0;
     x->ph.positivity = c.positivity ? 1 : 0;
-    x->flux = select_flux(c.recon_order, c.characteristic != 0, c.scheme);
+    x->ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;
+    x->flux = c.precision == RPFVM_PRECISION_MIXED
+                  ? select_flux_mixed(c.recon_order, c.characteristic != 0, c.scheme)
+                  : select_flux(c.recon_order, c.characteristic != 0, c.scheme);
     for (int s = 0; s < 4; s++) {
         BcSide& b = x->bc[s];
         b.side = s;
@@ -193,13 +200,16 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
 
     const size_t arr = 4 * x->g.plane + 32;             // doubles per state-sized array
     const size_t arr_bytes = ((arr * sizeof(double) + 255) / 256) * 256;
-    const int narr = c.positivity ? 7 : 6;
+    const bool separate_out = c.positivity || c.precision == RPFVM_PRECISION_MIXED;
+    const int narr = separate_out ? 7 : 6;
     const int nb = dt_blocks(x->g);
     const size_t small = 256 * 4 + (size_t)nb * 2 * sizeof(double) + 256;
     const size_t ncell = (size_t)c.nx * c.ny;
     const size_t cell_bytes = ((ncell * 4 + 255) / 256) * 256;
     const size_t rep_bytes = c.positivity ? 5 * cell_bytes + 256 : 0;
-    const size_t total = narr * arr_bytes + small + rep_bytes;
+    const size_t face_bytes = (((size_t)(c.nx + 1) * (c.ny + 1) * 4 + 255) / 256) * 256;
+    const size_t fix_bytes = c.precision == RPFVM_PRECISION_MIXED ? 2 * face_bytes + 256 : 0;
+    const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes;
     he = hipMalloc(&x->pool, total);
     if (he != hipSuccess)
         return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(he)));
@@ -221,6 +231,12 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
         x->rp.tch = reinterpret_cast<unsigned*>(r + 4 * cell_bytes);
         x->rp.nx = c.nx;
         x->rp.ny = c.ny;
+    }
+    if (fix_bytes) {
+        char* r = q + small + rep_bytes;
+        x->fix.cnt = reinterpret_cast<unsigned*>(r);
+        x->fix.list[0] = reinterpret_cast<int*>(r + 256);
+        x->fix.list[1] = reinterpret_cast<int*>(r + 256 + face_bytes);
     }
     he = hipMemsetAsync(x->pool, 0, total, x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
@@ -282,22 +298,18 @@ static void enqueue_bc(rpfvm_ctx* x, double* Y) {
 static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
     {
         Timed t(x, RPFVM_KCLASS_FLUX_X);
-        x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->stream);
+        x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
     }
     {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
-        x->flux.y(Y, x->G, x->g, x->ph, x->faults, x->stream);
+        x->flux.y(Y, x->G, x->g, x->ph, x->faults, x->fix, x->stream);
     }
 }
 
 static void enqueue_update(rpfvm_ctx* x, const double* Un, const double* Y, double* out,
-                           double dt, int kind, bool repair) {
+                           double dt, int kind, const Repair& rp) {
     Timed t(x, RPFVM_KCLASS_UPDATE);
-    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, repair ? x->rp : Repair{},
-                  x->stream);
-    if (repair && x->rp.cnt)
-        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, ++x->stamp, x->faults,
-                      x->stream);
+    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, rp, x->stream);
 }
 
 static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
@@ -305,19 +317,41 @@ static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
     launch_dt(x->U, x->g, x->ph, x->partial, x->scal, clock, x->faults, x->stream);
 }
 
-// One stage: ghost fill of Y, face fluxes, residual and stage combine into out.
+// One stage: ghost fill of Y, x-face fluxes, then the y faces fused with the residual
+// and stage combine into out (separately when out aliases Y, which the fused kernel's
+// neighbour reads forbid), then the positivity safeguard.
 static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
                           int kind) {
+    const unsigned stamp = ++x->stamp;
     enqueue_bc(x, Y);
-    enqueue_fluxes(x, Y);
-    enqueue_update(x, Un, Y, out, dt, kind, true);
+    {
+        Timed t(x, RPFVM_KCLASS_FLUX_X);
+        x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
+    }
+    if (out != Y) {
+        Timed t(x, RPFVM_KCLASS_FLUX_Y);
+        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, x->faults,
+                   x->stream);
+    } else {
+        {
+            Timed t(x, RPFVM_KCLASS_FLUX_Y);
+            x->flux.y(Y, x->G, x->g, x->ph, x->faults, x->fix, x->stream);
+        }
+        enqueue_update(x, Un, Y, out, dt, kind, x->rp);
+    }
+    if (x->rp.cnt) {
+        Timed t(x, RPFVM_KCLASS_UPDATE);
+        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, x->faults,
+                      x->stream);
+    }
 }
 
 static int enqueue_step(rpfvm_ctx* x, double dt) {
-    // With the positivity safeguard the last stage cannot update U in place (a repaired
-    // neighbour is recomputed from U^n): it writes U3, which takes over U's ghost frame
-    // (the reference leaves U's ghosts as filled at stage 1) and becomes the state.
-    double* last = x->rp.cnt ? x->U3 : x->U;
+    // With the positivity safeguard or the mixed form the last stage cannot update U in
+    // place (a repaired or FP64-recomputed cell is recomputed from U^n): it writes U3,
+    // which takes over U's ghost frame (the reference leaves U's ghosts as filled at
+    // stage 1) and becomes the state.
+    double* last = x->U3 ? x->U3 : x->U;
     switch (x->cfg.time_order) {
         case 1:
             enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER);
@@ -367,7 +401,7 @@ int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
     enqueue_bc(x, x->U);
     enqueue_fluxes(x, x->U);
-    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, false);
+    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, Repair{});
     HIPCHK(hipGetLastError());
     int e = 0;
     if (L && (e = copy_out(x, L, x->L))) return e;

@@ -1,0 +1,145 @@
+// fvm_fused.hpp -- y-face flux fused with the residual and RK stage combine.
+//
+// A 64-lane row of a block computes the y faces f = jb .. jb+63 of one grid row and
+// updates the 63 cells c = f + 1 between them: lane t owns face f = jb + t (the cell's
+// lower face, G_{c-1/2}) and receives the upper face G_{c+1/2} from lane t + 1 with a
+// wavefront shuffle, so the face fluxes never round-trip through memory before use (they
+// are still stored, for the positivity safeguard and the mixed form's FP64 recompute).
+// The cell arithmetic is exactly cell_update's (Space_Discrete_Item + stage combine), so
+// the fused path is bit-identical to k_flux<2> + k_update.
+#pragma once
+
+#include "fvm_kernels.hpp"
+#include "fvm_math.hpp"
+
+namespace fvm {
+
+constexpr int kStrip = 63;  // cells updated per 64-lane row
+
+// Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
+// fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872) for
+// cell (i, j).  `out` may alias `Un` (the last stage writes the state in place): each
+// thread reads its own cell before writing it.  Returns whether the forward-Euler
+// state W = Y + dt L has rho > 0 and p > 0 (used by the positivity safeguard).
+__device__ __forceinline__ bool cell_update(const double* Un, const double* Y, double* out,
+                                            const double* __restrict__ F,
+                                            const double* __restrict__ G, const Geom& g,
+                                            const Phys& ph, double dt, int kind, int i, int j) {
+    const size_t c = (size_t)i * g.ld + j;
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ph.source) {
+        const double rho = Y[c];
+        const double v = Y[2 * g.plane + c] / rho;
+        S[2] = rho * ph.gravity;
+        S[3] = rho * ph.gravity * v;
+    }
+    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+    double W[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const size_t o = (size_t)k * g.plane + c;
+        const double L = -(F[o] - F[o - g.ld]) * idx - (G[o] - G[o - 1]) * idy + S[k];
+        const double y = Y[o];
+        W[k] = y + dt * L;
+        double r;
+        switch (kind) {
+            case UPD_RESIDUAL: r = L; break;                                   // L only
+            case UPD_EULER: r = y + dt * L; break;                             // U1 = Y + dt L
+            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (y + dt * L); break;
+            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (y + dt * L); break;
+            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (y + dt * L); break;  // UPD_RK3_3
+        }
+        out[o] = r;
+    }
+    const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+    return admissible(kc, W);
+}
+
+
+// Residual + stage combine of cell (i, c) with the y-face fluxes given in registers
+// (gl = G_{c-1/2}, gu = G_{c+1/2}) and the cell's own state y[] (already loaded).
+// Returns admissibility of the forward-Euler state.
+__device__ __forceinline__ bool cell_update_regs(const double* Un, double* out,
+                                                 const double* __restrict__ F, const Geom& g,
+                                                 const Phys& ph, double dt, int kind, int i,
+                                                 int c, const double y[4], const double gl[4],
+                                                 const double gu[4]) {
+    const size_t o0 = (size_t)i * g.ld + c;
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ph.source) {
+        const double rho = y[0];
+        const double v = y[2] / rho;
+        S[2] = rho * ph.gravity;
+        S[3] = rho * ph.gravity * v;
+    }
+    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+    double W[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const size_t o = (size_t)k * g.plane + o0;
+        const double L = -(F[o] - F[o - g.ld]) * idx - (gu[k] - gl[k]) * idy + S[k];
+        W[k] = y[k] + dt * L;
+        double r;
+        switch (kind) {
+            case UPD_EULER: r = y[k] + dt * L; break;
+            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (y[k] + dt * L); break;
+            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (y[k] + dt * L); break;
+            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (y[k] + dt * L); break;
+        }
+        out[o] = r;
+    }
+    const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+    return admissible(kc, W);
+}
+
+// FaceOp::compute(st, f) returns false when the face must be recomputed later (mixed
+// form); FaceOp::defer(c0) records such a face.
+template <typename FaceOp>
+__device__ __forceinline__ void yflux_update_body(const double* Un, const double* Y, double* out,
+                                                  const double* __restrict__ F, double* G,
+                                                  const Geom& g, const Phys& ph, double dt,
+                                                  int kind, const Repair& rp, FaceOp& op) {
+    const int t = threadIdx.x;
+    const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
+    const int f = GC - 1 + blockIdx.x * kStrip + t;
+    const bool face_ok = i < g.R - GC && f <= g.C - GC - 1;
+    double gf[4] = {0.0, 0.0, 0.0, 0.0};
+    double y[4] = {0.0, 0.0, 0.0, 0.0};
+    int pending = 0;
+    if (face_ok) {
+        const size_t c0 = (size_t)i * g.ld + f;
+        double st[6][4];
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2)];
+#pragma unroll
+        for (int v = 0; v < 4; v++) y[v] = st[3][v];
+        if (op.compute(st, gf)) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) G[(size_t)v * g.plane + c0] = gf[v];
+        } else {
+            op.defer(c0);
+            pending = 1;
+        }
+    }
+    double gu[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
+    const int pending_up = __shfl_down(pending, 1, 64);
+    const int c = f + 1;
+    if (t < kStrip && face_ok && c <= g.C - GC - 1) {
+        const bool ok = cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu);
+        if (rp.cnt && !ok && !pending && !pending_up) {
+            const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+            rp.list[0][k] = (i - GC) * rp.ny + (c - GC);
+        }
+    }
+}
+
+inline dim3 yflux_update_grid(const Geom& g) {
+    const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
+    return dim3((nj + kStrip - 1) / kStrip, (ni + 3) / 4);
+}
+
+}  // namespace fvm

@@ -12,6 +12,7 @@
 // lanes walk the contiguous (y) index, so every load and store is coalesced.
 #include <hip/hip_runtime.h>
 
+#include "fvm_fused.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 
@@ -80,9 +81,10 @@ __global__ __launch_bounds__(256) void k_flux(const double* __restrict__ Y, doub
     for (int n = 0; n < 6; n++)
 #pragma unroll
         for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2) * step];
+    const Consts kc = make_consts(ph);
     double UL[4], UR[4];
-    reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
-    if (ph.positivity && !(admissible(ph.gamma, UL) && admissible(ph.gamma, UR))) {
+    reconstruct_face<DIR, ORDER, CHAR>(kc, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.positivity && !(admissible(kc, UL) && admissible(kc, UR))) {
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             UL[v] = st[2][v];
@@ -90,50 +92,45 @@ __global__ __launch_bounds__(256) void k_flux(const double* __restrict__ Y, doub
         }
     }
     double f[4];
-    if (riemann<DIR, SOLVER>(ph.gamma, UL, UR, f)) atomicOr(faults, 1u);
+    if (riemann<DIR, SOLVER>(kc, UL, UR, f)) atomicOr(faults, 1u);
 #pragma unroll
     for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
 }
 
-// ------------------------------------------------------------------ update
-// Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
-// fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872) for
-// cell (i, j).  `out` may alias `Un` (the last stage writes the state in place): each
-// thread reads its own cell before writing it.  Returns whether the forward-Euler
-// state W = Y + dt L has rho > 0 and p > 0 (used by the positivity safeguard).
-__device__ __forceinline__ bool cell_update(const double* Un, const double* Y, double* out,
-                                            const double* __restrict__ F,
-                                            const double* __restrict__ G, const Geom& g,
-                                            const Phys& ph, double dt, int kind, int i, int j) {
-    const size_t c = (size_t)i * g.ld + j;
-    double S[4] = {0.0, 0.0, 0.0, 0.0};
-    if (ph.source) {
-        const double rho = Y[c];
-        const double v = Y[2 * g.plane + c] / rho;
-        S[2] = rho * ph.gravity;
-        S[3] = rho * ph.gravity * v;
-    }
-    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
-    double W[4];
+// FP64 y faces fused with the cell update (fvm_fused.hpp).
+template <int ORDER, bool CHAR, int SOLVER>
+struct FaceOp64 {
+    const Phys& ph;
+    Consts kc;
+    unsigned* faults;
+    __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
+        double UL[4], UR[4];
+        reconstruct_face<2, ORDER, CHAR>(kc, ph.dy, st, UL, UR);
+        if (ph.positivity && !(admissible(kc, UL) && admissible(kc, UR))) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const size_t o = (size_t)k * g.plane + c;
-        const double L = -(F[o] - F[o - g.ld]) * idx - (G[o] - G[o - 1]) * idy + S[k];
-        const double y = Y[o];
-        W[k] = y + dt * L;
-        double r;
-        switch (kind) {
-            case UPD_RESIDUAL: r = L; break;                                   // L only
-            case UPD_EULER: r = y + dt * L; break;                             // U1 = Y + dt L
-            case UPD_RK2_2: r = 0.5 * Un[o] + 0.5 * (y + dt * L); break;
-            case UPD_RK3_2: r = (3.0 / 4.0) * Un[o] + (1.0 / 4.0) * (y + dt * L); break;
-            default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (y + dt * L); break;  // UPD_RK3_3
+            for (int v = 0; v < 4; v++) {
+                UL[v] = st[2][v];
+                UR[v] = st[3][v];
+            }
         }
-        out[o] = r;
+        if (riemann<2, SOLVER>(kc, UL, UR, f)) atomicOr(faults, 1u);
+        return true;
     }
-    return admissible(ph.gamma, W);
+    __device__ __forceinline__ void defer(size_t) {}
+};
+
+template <int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_yflux_update(const double* Un, const double* Y,
+                                                      double* out, const double* __restrict__ F,
+                                                      double* G, Geom g, Phys ph, double dt,
+                                                      int kind, Repair rp,
+                                                      unsigned* __restrict__ faults) {
+    FaceOp64<ORDER, CHAR, SOLVER> op{ph, make_consts(ph), faults};
+    yflux_update_body(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
 }
 
+// ------------------------------------------------------------------ update
+// k_update: cell_update (fvm_fused.hpp) over the interior.
 __global__ __launch_bounds__(256) void k_update(const double* Un, const double* Y, double* out,
                                                 const double* __restrict__ F,
                                                 const double* __restrict__ G, Geom g, Phys ph,
@@ -180,7 +177,8 @@ __global__ __launch_bounds__(256) void k_repair_faces(const double* __restrict__
                 a[k] = Y[(size_t)k * g.plane + c];
                 b[k] = Y[(size_t)k * g.plane + nb];
             }
-            const int bad = side < 2 ? hlle<1>(ph.gamma, a, b, f) : hlle<2>(ph.gamma, a, b, f);
+            const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+            const int bad = side < 2 ? hlle<1>(kc, a, b, f) : hlle<2>(kc, a, b, f);
             if (bad) atomicOr(faults, 1u);
             double* D = side < 2 ? F : G;
 #pragma unroll
@@ -340,7 +338,7 @@ __global__ void k_dt_final(const double* __restrict__ partial, int nblocks, Phys
 // ----------------------------------------------------------------- launchers
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
 static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& ph,
-                        unsigned* faults, hipStream_t s) {
+                        unsigned* faults, const FaceFix&, hipStream_t s) {
     const int nj = DIR == 1 ? g.C - 2 * GC : g.C - 2 * GC + 1;
     const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
     dim3 block(64, 4);
@@ -349,8 +347,18 @@ static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& p
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
+static void launch_yflux_update(const double* Un, const double* Y, double* out, const double* F,
+                                double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                                const Repair& rp, const FaceFix&, unsigned, unsigned* faults,
+                                hipStream_t s) {
+    hipLaunchKernelGGL((k_yflux_update<ORDER, CHAR, SOLVER>), yflux_update_grid(g), dim3(64, 4), 0,
+                       s, Un, Y, out, F, G, g, ph, dt, kind, rp, faults);
+}
+
+template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair pick3() {
-    return FluxPair{&launch_flux<1, ORDER, CHAR, SOLVER>, &launch_flux<2, ORDER, CHAR, SOLVER>};
+    return FluxPair{&launch_flux<1, ORDER, CHAR, SOLVER>, &launch_flux<2, ORDER, CHAR, SOLVER>,
+                    &launch_yflux_update<ORDER, CHAR, SOLVER>};
 }
 
 template <int ORDER, bool CHAR>

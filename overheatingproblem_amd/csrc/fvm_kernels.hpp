@@ -19,7 +19,9 @@ struct Geom {
 // Physical and scheme parameters read by the kernels.
 struct Phys {
     double gamma, dx, dy, cfl, gravity;
+    double gm1, gr, six;  // gamma - 1, gamma / (gamma - 1), 6.0 (host-computed)
     int source, positivity;
+    int force_fix;  // mixed form: send every face to the FP64 recompute list
 };
 
 enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4 };
@@ -45,13 +47,27 @@ struct Repair {
 };
 constexpr int kRepairRounds = 8;
 
+// Faces of the mixed form that need the FP64 path (x and y lists, reset alternately).
+struct FaceFix {
+    unsigned* cnt;  // [0] x-face count, [1] y-face count
+    int* list[2];   // face offsets i*ld + j
+};
+
 using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phys& ph,
-                            unsigned* faults, hipStream_t s);
+                            unsigned* faults, const FaceFix& fix, hipStream_t s);
+// y faces fused with the residual + stage combine (out must not alias Y); `stamp` is
+// the stage's unique stamp for de-duplicating safeguard list entries.
+using FusedLaunch = void (*)(const double* Un, const double* Y, double* out, const double* F,
+                             double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                             const Repair& rp, const FaceFix& fix, unsigned stamp,
+                             unsigned* faults, hipStream_t s);
 struct FluxPair {
     FluxLaunch x, y;
+    FusedLaunch yu;
 };
 
 FluxPair select_flux(int order, bool characteristic, int solver);
+FluxPair select_flux_mixed(int order, bool characteristic, int solver);
 void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st);
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
                    const double* G, const Geom& g, const Phys& ph, double dt, int kind,
@@ -59,6 +75,10 @@ void launch_update(const double* Un, const double* Y, double* out, const double*
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
                    const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
                    unsigned stamp, unsigned* faults, hipStream_t s);
+// Cells next to the y faces the mixed form recomputed in FP64 (fvm_mixed_kernels.hip).
+void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
+                       const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                       const Repair& rp, const FaceFix& fix, unsigned stamp, hipStream_t s);
 void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s);
 int dt_blocks(const Geom& g);
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,

@@ -12,6 +12,45 @@
 
 namespace fvm {
 
+// ------------------------------------------------------ shared-denominator division
+// gfx950's IEEE FP64 division a/b is the sequence
+//   d = div_scale(b), n = div_scale(a), y0 = rcp(d), y1 = y0 + y0(1 - d y0),
+//   y2 = y1 + y1(1 - d y1), q = n y2, r = n - d q, div_fmas(r, y2, q), div_fixup.
+// div_scale rescales only near the exponent limits (exponent difference >= 768,
+// numerator within 2^53 of the denormals, denominator or quotient denormal), and
+// div_fixup only changes special or out-of-range results.  For |a|, |b| in
+// [2^-300, 2^300] neither applies, so a/b == fma(fma(-b, a*y2, a), y2, a*y2) exactly.
+// Recip holds y2 of one denominator; quot() finishes each numerator with 3 FP64 ops
+// (and falls back to a true division outside the safe range), so quotients sharing a
+// denominator cost one reciprocal refinement instead of one full sequence each.
+struct Recip {
+    double b, y;
+    bool ok;
+};
+
+__device__ __forceinline__ bool in_safe_range(double x) {
+    const double ax = fabs(x);
+    return ax >= 0x1p-300 && ax <= 0x1p300;
+}
+
+__device__ __forceinline__ Recip recip(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    return Recip{b, y, in_safe_range(b)};
+}
+
+__device__ __forceinline__ double quot(double a, const Recip& r) {
+    if (r.ok && in_safe_range(a)) {
+        const double q = a * r.y;
+        const double rem = fma(-r.b, q, a);
+        return fma(rem, r.y, q);
+    }
+    return a / r.b;
+}
+
 // ------------------------------------------------------------------- helpers
 // sgn / max_of_two / min_of_two / min_mod / van_leer: include/function.h:31-74
 __device__ __forceinline__ int sgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
@@ -41,37 +80,44 @@ __device__ __forceinline__ Beta5 beta5(double v1, double v2, double v3, double v
     return r;
 }
 
-// Weighted combination of the three third-order candidates, :1567-1569.
+// Weighted combination of the three third-order candidates, :1567-1569.  r6 is the
+// Recip of 6.0 (built from a runtime value so the compiler cannot fold the hardware
+// reciprocal estimate).
 __device__ __forceinline__ double weno5_combine(double a1, double a2, double a3, double v1,
-                                                double v2, double v3, double v4, double v5) {
-    const double w1 = a1 / (a1 + a2 + a3);
-    const double w2 = a2 / (a1 + a2 + a3);
-    const double w3 = a3 / (a1 + a2 + a3);
-    return w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3) / 6.0
-         + w2 * (-v2 + 5.0 * v3 + 2.0 * v4) / 6.0
-         + w3 * (2.0 * v3 + 5.0 * v4 - v5) / 6.0;
+                                                double v2, double v3, double v4, double v5,
+                                                const Recip& r6) {
+    const Recip rs = recip(a1 + a2 + a3);
+    const double w1 = quot(a1, rs);
+    const double w2 = quot(a2, rs);
+    const double w3 = quot(a3, rs);
+    return quot(w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3), r6)
+         + quot(w2 * (-v2 + 5.0 * v3 + 2.0 * v4), r6)
+         + quot(w3 * (2.0 * v3 + 5.0 * v4 - v5), r6);
 }
 
 // WENO5-Z (Borges et al. 2008, exponent 1, eps 1e-15): WENO5Z_L/R,
 // include/Reconstruction.h:1519-1624.  (v1..v5) in upwind order.
-__device__ __forceinline__ double weno5z(double v1, double v2, double v3, double v4, double v5) {
+__device__ __forceinline__ double weno5z(double v1, double v2, double v3, double v4, double v5,
+                                         const Recip& r6) {
     const double eps = 1.0e-15;
     const Beta5 b = beta5(v1, v2, v3, v4, v5);
     const double tau = fabs(b.b1 - b.b3);
     const double a1 = 0.1 * (1.0 + tau / (b.b1 + eps));
     const double a2 = 0.6 * (1.0 + tau / (b.b2 + eps));
     const double a3 = 0.3 * (1.0 + tau / (b.b3 + eps));
-    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5);
+    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5, r6);
 }
 
 // WENO5-JS: WENO5_L/R, include/Reconstruction.h:1408-1508.
-__device__ __forceinline__ double weno5js(double v1, double v2, double v3, double v4, double v5) {
+__device__ __forceinline__ double weno5js(double v1, double v2, double v3, double v4, double v5,
+                                          const Recip& r6) {
     const double eps = 1.0e-15;
     const Beta5 b = beta5(v1, v2, v3, v4, v5);
-    const double a1 = 0.1 / (eps + b.b1) / (eps + b.b1);
-    const double a2 = 0.6 / (eps + b.b2) / (eps + b.b2);
-    const double a3 = 0.3 / (eps + b.b3) / (eps + b.b3);
-    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5);
+    const Recip r1 = recip(eps + b.b1), r2 = recip(eps + b.b2), r3 = recip(eps + b.b3);
+    const double a1 = quot(quot(0.1, r1), r1);
+    const double a2 = quot(quot(0.6, r2), r2);
+    const double a3 = quot(quot(0.3, r3), r3);
+    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5, r6);
 }
 
 // WENO3_L/R, include/Reconstruction.h:1326-1398 (pow(x,2) folded to x*x).
@@ -99,8 +145,8 @@ __device__ __forceinline__ double tvd_slope(double v1, double v2, double v3, dou
 // order ORDER; CHAR selects WENO5-Z / van Leer (characteristic branch) over
 // WENO5-JS / minmod (component branch).
 template <int ORDER, bool CHAR>
-__device__ __forceinline__ void recon_scalar(const double s[6], double delta, double& left,
-                                             double& right) {
+__device__ __forceinline__ void recon_scalar(const double s[6], double delta, const Recip& r6,
+                                             double& left, double& right) {
     if constexpr (ORDER == 1) {
         left = s[2];
         right = s[3];
@@ -114,48 +160,70 @@ __device__ __forceinline__ void recon_scalar(const double s[6], double delta, do
         right = weno3(s[4], s[3], s[2]);
     } else {
         if constexpr (CHAR) {
-            left = weno5z(s[0], s[1], s[2], s[3], s[4]);
-            right = weno5z(s[5], s[4], s[3], s[2], s[1]);
+            left = weno5z(s[0], s[1], s[2], s[3], s[4], r6);
+            right = weno5z(s[5], s[4], s[3], s[2], s[1], r6);
         } else {
-            left = weno5js(s[0], s[1], s[2], s[3], s[4]);
-            right = weno5js(s[5], s[4], s[3], s[2], s[1]);
+            left = weno5js(s[0], s[1], s[2], s[3], s[4], r6);
+            right = weno5js(s[5], s[4], s[3], s[2], s[1], r6);
         }
     }
 }
 
 // ------------------------------------------------------------ eigenvectors
-// Con_to_Pri_2D for one cell, include/initialize.h:1443-1451.
-__device__ __forceinline__ void primitive(double g, const double c[4], double p[4]) {
+// Uniform constants of a kernel: gm1 = gamma - 1 and gr = gamma / (gamma - 1) come from
+// the host (IEEE division rounds identically there), r6 is the Recip of 6.0.
+struct Consts {
+    double g, gm1, gr;
+    Recip r6;
+};
+
+__device__ __forceinline__ Consts make_consts(const Phys& ph) {
+    return Consts{ph.gamma, ph.gm1, ph.gr, recip(ph.six)};
+}
+
+// Con_to_Pri_2D for one cell (include/initialize.h:1443-1451) and the enthalpy
+// Compute_Eigen_2DX/2DY derive from it (include/Characteriz.h:318,420); the three
+// quotients by rho share one reciprocal.
+__device__ __forceinline__ void primitive_h(const Consts& k, const double c[4], double p[4],
+                                            double& H, Recip& rr) {
+    rr = recip(c[0]);
     const double rho = c[0];
-    const double u = c[1] / c[0];
-    const double v = c[2] / c[0];
+    const double u = quot(c[1], rr);
+    const double v = quot(c[2], rr);
     p[0] = rho;
     p[1] = u;
     p[2] = v;
-    p[3] = (c[3] - 0.5 * rho * (u * u + v * v)) * (g - 1.0);
+    p[3] = (c[3] - 0.5 * rho * (u * u + v * v)) * k.gm1;
+    H = 0.5 * (u * u + v * v) + k.gr * quot(p[3], rr);
 }
 
 // The left (EL[wave][var]) and right (ER[var][wave]) eigenvectors of the face between
-// primitive states a and b.  DIR 1: Compute_Eigen_2DX (include/Characteriz.h:300-398);
-// DIR 2: Compute_Eigen_2DY (:402-500), with its literal 0.0/c and -0.0 terms.
+// cells a and b (conserved states).  DIR 1: Compute_Eigen_2DX (include/Characteriz.h:
+// 300-398); DIR 2: Compute_Eigen_2DY (:402-500), with its literal 0.0/c and -0.0 terms
+// (0.0/c == 0.0 * (1/c) for every c = sqrt(c2) >= +0 or NaN).
 template <int DIR>
-__device__ __forceinline__ void face_eigen(double g, const double a[4], const double b[4],
+__device__ __forceinline__ void face_eigen(const Consts& k, const double ca[4], const double cb[4],
                                            double EL[4][4], double ER[4][4]) {
-    const double Ha = 0.5 * (a[1] * a[1] + a[2] * a[2]) + (g / (g - 1.0)) * (a[3] / a[0]);
-    const double Hb = 0.5 * (b[1] * b[1] + b[2] * b[2]) + (g / (g - 1.0)) * (b[3] / b[0]);
-    const double D = sqrt(b[0] / a[0]);
+    double a[4], b[4], Ha, Hb;
+    Recip ra, rb;
+    primitive_h(k, ca, a, Ha, ra);
+    primitive_h(k, cb, b, Hb, rb);
+    const double D = sqrt(quot(b[0], ra));
     const double D1 = D + 1.0;
-    const double u = (a[1] + D * b[1]) / D1;
-    const double v = (a[2] + D * b[2]) / D1;
-    const double H = (Ha + D * Hb) / D1;
+    const Recip rd = recip(D1);
+    const double u = quot(a[1] + D * b[1], rd);
+    const double v = quot(a[2] + D * b[2], rd);
+    const double H = quot(Ha + D * Hb, rd);
     const double q2 = u * u + v * v;
-    const double c2 = (g - 1.0) * (H - 0.5 * q2);
+    const double c2 = k.gm1 * (H - 0.5 * q2);
     const double c = sqrt(c2);
-    const double b1 = (g - 1.0) / c2;
+    const double b1 = k.gm1 / c2;
     const double b2 = 1.0 + b1 * q2 - b1 * H;
-    const double ic = 1.0 / c;
+    const Recip rc = recip(c);
+    const double ic = quot(1.0, rc);
     if constexpr (DIR == 1) {
-        EL[0][0] = 0.5 * (b2 + u / c);
+        const double uc = quot(u, rc);
+        EL[0][0] = 0.5 * (b2 + uc);
         EL[0][1] = -0.5 * (b1 * u + ic);
         EL[0][2] = -0.5 * b1 * v;
         EL[0][3] = 0.5 * b1;
@@ -167,7 +235,7 @@ __device__ __forceinline__ void face_eigen(double g, const double a[4], const do
         EL[2][1] = 0.0;
         EL[2][2] = -1.0;
         EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - u / c);
+        EL[3][0] = 0.5 * (b2 - uc);
         EL[3][1] = 0.5 * (-b1 * u + ic);
         EL[3][2] = 0.5 * -b1 * v;
         EL[3][3] = 0.5 * b1;
@@ -188,8 +256,9 @@ __device__ __forceinline__ void face_eigen(double g, const double a[4], const do
         ER[3][2] = -v;
         ER[3][3] = H + u * c;
     } else {
-        const double zc = 0.0 / c;
-        EL[0][0] = 0.5 * (b2 + v / c);
+        const double vc = quot(v, rc);
+        const double zc = 0.0 * ic;
+        EL[0][0] = 0.5 * (b2 + vc);
         EL[0][1] = -0.5 * (b1 * u + zc);
         EL[0][2] = -0.5 * (b1 * v + ic);
         EL[0][3] = 0.5 * b1;
@@ -201,7 +270,7 @@ __device__ __forceinline__ void face_eigen(double g, const double a[4], const do
         EL[2][1] = 1.0;
         EL[2][2] = -0.0;
         EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - v / c);
+        EL[3][0] = 0.5 * (b2 - vc);
         EL[3][1] = 0.5 * (-b1 * u + zc);
         EL[3][2] = 0.5 * (-b1 * v + ic);
         EL[3][3] = 0.5 * b1;
@@ -229,13 +298,12 @@ __device__ __forceinline__ void face_eigen(double g, const double a[4], const do
 // U = R.w (sum from 0.0 in wave order): include/Reconstruction.h:909-967 and its
 // Godunov/TVD/WENO3 twins; component branch: :1031-1073 and twins.
 template <int DIR, int ORDER, bool CHAR>
-__device__ __forceinline__ void reconstruct_face(double g, double delta, const double st[6][4],
-                                                 double UL[4], double UR[4]) {
+__device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
+                                                 const double st[6][4], double UL[4],
+                                                 double UR[4]) {
     if constexpr (CHAR) {
-        double pa[4], pb[4], EL[4][4], ER[4][4];
-        primitive(g, st[2], pa);
-        primitive(g, st[3], pb);
-        face_eigen<DIR>(g, pa, pb, EL, ER);
+        double EL[4][4], ER[4][4];
+        face_eigen<DIR>(kc, st[2], st[3], EL, ER);
         double wl[4], wr[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -257,7 +325,7 @@ __device__ __forceinline__ void reconstruct_face(double g, double delta, const d
                     for (int m = 0; m < 4; m++) acc += st[n][m] * EL[k][m];
                     w[n] = acc;
                 }
-                recon_scalar<ORDER, true>(w, delta, wl[k], wr[k]);
+                recon_scalar<ORDER, true>(w, delta, kc.r6, wl[k], wr[k]);
             }
         }
 #pragma unroll
@@ -275,7 +343,7 @@ __device__ __forceinline__ void reconstruct_face(double g, double delta, const d
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const double s[6] = {st[0][v], st[1][v], st[2][v], st[3][v], st[4][v], st[5][v]};
-            recon_scalar<ORDER, false>(s, delta, UL[v], UR[v]);
+            recon_scalar<ORDER, false>(s, delta, kc.r6, UL[v], UR[v]);
         }
     }
 }
@@ -285,7 +353,7 @@ struct Side {
     double r, ru, rv, E, ir, u, v, u2, v2, p;
 };
 
-__device__ __forceinline__ Side make_side(double g, const double c[4]) {
+__device__ __forceinline__ Side make_side(const Consts& k, const double c[4]) {
     Side s;
     s.r = c[0];
     s.ru = c[1];
@@ -296,7 +364,7 @@ __device__ __forceinline__ Side make_side(double g, const double c[4]) {
     s.v = s.rv * s.ir;
     s.u2 = s.u * s.u;
     s.v2 = s.v * s.v;
-    s.p = (s.E - 0.5 * s.r * (s.u2 + s.v2)) * (g - 1.0);
+    s.p = (s.E - 0.5 * s.r * (s.u2 + s.v2)) * k.gm1;
     return s;
 }
 
@@ -318,10 +386,9 @@ __device__ __forceinline__ void phys_flux(const Side& s, double H, double F[4]) 
 // HLLC with Einfeldt speeds, include/scheme.h:690-1001 (per-direction round-off:
 // sound speed :745-746 vs :895-896, Einfeldt eta :777 vs :929).
 template <int DIR>
-__device__ __forceinline__ int hllc(double g, const double cl[4], const double cr[4], double F[4]) {
-    const double gm1 = g - 1.0;
-    const double gr = g / gm1;
-    const Side L = make_side(g, cl), R = make_side(g, cr);
+__device__ __forceinline__ int hllc(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    const double g = k.g, gm1 = k.gm1, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     double aL, aR;
@@ -389,14 +456,14 @@ __device__ __forceinline__ int hllc(double g, const double cl[4], const double c
 // skips a face with rho <= 0, NaN rho (:427-436) or a NaN speed (:500-503), which
 // leaves stale memory in its flux array; here such a face gets NaN and a fault bit.
 template <int DIR>
-__device__ __forceinline__ int hll(double g, const double cl[4], const double cr[4], double F[4]) {
+__device__ __forceinline__ int hll(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
     if (cl[0] <= 0.0 || cr[0] <= 0.0 || isnan(cl[0]) || isnan(cr[0])) {
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
         return 1;
     }
-    const Side L = make_side(g, cl), R = make_side(g, cr);
-    const double gr = g / (g - 1.0);
+    const double g = k.g, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     const double aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
@@ -437,14 +504,14 @@ __device__ __forceinline__ int hll(double g, const double cl[4], const double cr
 // averages.  Used only by the positivity safeguard (an extension): the reference's HLL
 // uses the unbounded speeds q~ -/+ d, which do not preserve positivity.
 template <int DIR>
-__device__ __forceinline__ int hlle(double g, const double cl[4], const double cr[4], double F[4]) {
+__device__ __forceinline__ int hlle(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
     if (!(cl[0] > 0.0) || !(cr[0] > 0.0)) {
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
         return 1;
     }
-    const Side L = make_side(g, cl), R = make_side(g, cr);
-    const double gr = g / (g - 1.0);
+    const double g = k.g, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     const double aL = sqrt(fmax(g * L.p * L.ir, 0.0));
@@ -478,10 +545,9 @@ __device__ __forceinline__ int hlle(double g, const double cl[4], const double c
 
 // Roe with the reference's no-op entropy fix, include/scheme.h:1018-1322.
 template <int DIR>
-__device__ __forceinline__ int roe(double g, const double cl[4], const double cr[4], double F[4]) {
-    const double gm1 = g - 1.0;
-    const double gr = g / gm1;
-    const Side L = make_side(g, cl), R = make_side(g, cr);
+__device__ __forceinline__ int roe(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    const double g = k.g, gm1 = k.gm1, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     double FL[4], FR[4];
@@ -530,12 +596,11 @@ __device__ __forceinline__ int roe(double g, const double cl[4], const double cr
 
 // Local Lax-Friedrichs (the reference's default solver branch), include/scheme.h:229-381.
 template <int DIR>
-__device__ __forceinline__ int llf(double g, const double cl[4], const double cr[4], double F[4]) {
-    const double gm1 = g - 1.0;
-    const Side L = make_side(g, cl), R = make_side(g, cr);
+__device__ __forceinline__ int llf(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    const double g = k.g, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
     const double aL = sqrt(g * L.p * L.ir);
     const double aR = sqrt(g * R.p * R.ir);
-    const double gr = g / gm1;
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     double FL[4], FR[4];
@@ -551,19 +616,19 @@ __device__ __forceinline__ int llf(double g, const double cl[4], const double cr
 enum : int { SOLVER_HLL = 1, SOLVER_HLLC = 2, SOLVER_ROE = 3, SOLVER_LLF = 10 };
 
 template <int DIR, int SOLVER>
-__device__ __forceinline__ int riemann(double g, const double cl[4], const double cr[4], double F[4]) {
-    if constexpr (SOLVER == SOLVER_HLL) return hll<DIR>(g, cl, cr, F);
-    else if constexpr (SOLVER == SOLVER_HLLC) return hllc<DIR>(g, cl, cr, F);
-    else if constexpr (SOLVER == SOLVER_ROE) return roe<DIR>(g, cl, cr, F);
-    else return llf<DIR>(g, cl, cr, F);
+__device__ __forceinline__ int riemann(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    if constexpr (SOLVER == SOLVER_HLL) return hll<DIR>(k, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLC) return hllc<DIR>(k, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_ROE) return roe<DIR>(k, cl, cr, F);
+    else return llf<DIR>(k, cl, cr, F);
 }
 
 // Positivity test of a face state (rho > 0, p > 0, finite); used only by the
 // optional positivity safeguard, which the reference does not have.
-__device__ __forceinline__ bool admissible(double g, const double c[4]) {
+__device__ __forceinline__ bool admissible(const Consts& k, const double c[4]) {
     const double ir = 1.0 / c[0];
     const double u = c[1] * ir, v = c[2] * ir;
-    const double p = (c[3] - 0.5 * c[0] * (u * u + v * v)) * (g - 1.0);
+    const double p = (c[3] - 0.5 * c[0] * (u * u + v * v)) * k.gm1;
     return c[0] > 0.0 && p > 0.0 && isfinite(p) && isfinite(u) && isfinite(v);
 }
 

@@ -1,0 +1,203 @@
+// fvm_mixed_kernels.hip -- face-flux kernels of the mixed-precision form.
+//
+// Same launch geometry and output as k_flux (fvm_kernels.hip); the face arithmetic is
+// fvm::mx (fvm_mixed.hpp).  A face whose FP32-reconstructed states are not admissible
+// (rho <= 0 or p <= 0 or not finite) is appended to a list instead, and k_flux_fix
+// recomputes the listed faces entirely with the FP64 path right after (with the
+// positivity safeguard on, a face still inadmissible there falls back to first order).
+// Keeping the rare FP64 path out of the hot kernel keeps its registers and occupancy.
+#include <hip/hip_runtime.h>
+
+#include "fvm_fused.hpp"
+#include "fvm_kernels.hpp"
+#include "fvm_math.hpp"
+#include "fvm_mixed.hpp"
+
+namespace fvm {
+
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+__device__ __forceinline__ int face_fp64(const Phys& ph, const double st[6][4], double f[4]) {
+    const Consts kc = make_consts(ph);
+    double UL[4], UR[4];
+    reconstruct_face<DIR, ORDER, CHAR>(kc, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.positivity && !(admissible(kc, UL) && admissible(kc, UR))) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            UL[v] = st[2][v];
+            UR[v] = st[3][v];
+        }
+    }
+    return riemann<DIR, SOLVER>(kc, UL, UR, f);
+}
+
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux_mixed(const double* __restrict__ Y,
+                                                    double* __restrict__ F, Geom g, Phys ph,
+                                                    FaceFix fix) {
+    const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
+    const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.R - GC || j >= g.C - GC) return;
+    const size_t c0 = (size_t)i * g.ld + j;
+    const size_t step = DIR == 1 ? (size_t)g.ld : 1;
+    double st[6][4];
+#pragma unroll
+    for (int n = 0; n < 6; n++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2) * step];
+    double UL[4], UR[4], f[4];
+    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR))) {
+        const unsigned k = atomicAdd(&fix.cnt[DIR - 1], 1u);
+        fix.list[DIR - 1][k] = (int)c0;
+        return;
+    }
+    mx::riemann<DIR, SOLVER>(ph.gamma, UL, UR, f);
+#pragma unroll
+    for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+}
+
+// FP64 recomputation of the listed faces; also resets the other direction's list,
+// which the next flux kernel (the other direction) refills.
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
+                                                  double* __restrict__ F, Geom g, Phys ph,
+                                                  FaceFix fix, unsigned* __restrict__ faults) {
+    const unsigned n = fix.cnt[DIR - 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        fix.cnt[2 - DIR] = 0;
+        if (n) atomicOr(faults, 4u);
+    }
+    const size_t step = DIR == 1 ? (size_t)g.ld : 1;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const size_t c0 = (size_t)fix.list[DIR - 1][q];
+        double st[6][4], f[4];
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (m - 2) * step];
+        if (face_fp64<DIR, ORDER, CHAR, SOLVER>(ph, st, f)) atomicOr(faults, 1u);
+#pragma unroll
+        for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+    }
+}
+
+// Mixed-form y faces fused with the cell update (fvm_fused.hpp).  A face that needs the
+// FP64 path is deferred to fix.list[1]; its two cells are updated again by
+// k_yfix_cells once k_flux_fix<2> has recomputed it.
+template <int ORDER, bool CHAR, int SOLVER>
+struct FaceOpMx {
+    const Phys& ph;
+    FaceFix fix;
+    __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
+        double UL[4], UR[4];
+        mx::reconstruct_face<2, ORDER, CHAR>(ph.gamma, ph.dy, st, UL, UR);
+        if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
+            return false;
+        mx::riemann<2, SOLVER>(ph.gamma, UL, UR, f);
+        return true;
+    }
+    __device__ __forceinline__ void defer(size_t c0) {
+        const unsigned k = atomicAdd(&fix.cnt[1], 1u);
+        fix.list[1][k] = (int)c0;
+    }
+};
+
+template <int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_yflux_update_mixed(const double* Un, const double* Y,
+                                                            double* out,
+                                                            const double* __restrict__ F,
+                                                            double* G, Geom g, Phys ph, double dt,
+                                                            int kind, Repair rp, FaceFix fix) {
+    FaceOpMx<ORDER, CHAR, SOLVER> op{ph, fix};
+    yflux_update_body(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
+}
+
+// Cells on either side of each recomputed y face: residual + stage combine again, with
+// the final G.  A cell next to two such faces is computed twice with identical results;
+// the positivity list receives it once (stamp de-duplication).
+__global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const double* Y, double* out,
+                                                    const double* __restrict__ F,
+                                                    const double* __restrict__ G, Geom g, Phys ph,
+                                                    double dt, int kind, Repair rp, FaceFix fix,
+                                                    unsigned tstamp) {
+    const unsigned n = fix.cnt[1];
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n; q += gridDim.x * blockDim.x) {
+        const size_t c0 = (size_t)fix.list[1][q >> 1];
+        const int i = (int)(c0 / g.ld);
+        const int j = (int)(c0 % g.ld) + (int)(q & 1);
+        if (j < GC || j > g.C - GC - 1) continue;
+        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+        if (rp.cnt && !ok) {
+            const int cell = (i - GC) * rp.ny + (j - GC);
+            if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
+                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+                rp.list[0][k] = cell;
+            }
+        }
+    }
+}
+
+void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
+                       const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                       const Repair& rp, const FaceFix& fix, unsigned stamp, hipStream_t s) {
+    hipLaunchKernelGGL(k_yfix_cells, dim3(64), dim3(256), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
+                       rp, fix, stamp * 16u + 15u);
+}
+
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_yflux_update_mixed(const double* Un, const double* Y, double* out,
+                                      const double* F, double* G, const Geom& g, const Phys& ph,
+                                      double dt, int kind, const Repair& rp, const FaceFix& fix,
+                                      unsigned stamp, unsigned* faults, hipStream_t s) {
+    hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER>), yflux_update_grid(g),
+                       dim3(64, 4), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix);
+    hipLaunchKernelGGL((k_flux_fix<2, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, G, g,
+                       ph, fix, faults);
+    launch_yfix_cells(Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, s);
+}
+
+template <int DIR, int ORDER, bool CHAR, int SOLVER>
+static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const Phys& ph,
+                              unsigned* faults, const FaceFix& fix, hipStream_t s) {
+    const int nj = DIR == 1 ? g.C - 2 * GC : g.C - 2 * GC + 1;
+    const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
+    dim3 block(64, 4);
+    dim3 grid((nj + 63) / 64, (ni + 3) / 4);
+    hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER>), grid, block, 0, s, Y, F, g, ph,
+                       fix);
+    hipLaunchKernelGGL((k_flux_fix<DIR, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, g,
+                       ph, fix, faults);
+}
+
+template <int ORDER, bool CHAR, int SOLVER>
+static FluxPair mpick3() {
+    return FluxPair{&launch_flux_mixed<1, ORDER, CHAR, SOLVER>,
+                    &launch_flux_mixed<2, ORDER, CHAR, SOLVER>,
+                    &launch_yflux_update_mixed<ORDER, CHAR, SOLVER>};
+}
+
+template <int ORDER, bool CHAR>
+static FluxPair mpick2(int solver) {
+    switch (solver) {
+        case SOLVER_HLL: return mpick3<ORDER, CHAR, SOLVER_HLL>();
+        case SOLVER_HLLC: return mpick3<ORDER, CHAR, SOLVER_HLLC>();
+        case SOLVER_ROE: return mpick3<ORDER, CHAR, SOLVER_ROE>();
+        default: return mpick3<ORDER, CHAR, SOLVER_LLF>();
+    }
+}
+
+template <int ORDER>
+static FluxPair mpick1(bool ch, int solver) {
+    return ch ? mpick2<ORDER, true>(solver) : mpick2<ORDER, false>(solver);
+}
+
+FluxPair select_flux_mixed(int order, bool characteristic, int solver) {
+    switch (order) {
+        case 1: return mpick1<1>(characteristic, solver);
+        case 2: return mpick1<2>(characteristic, solver);
+        case 3: return mpick1<3>(characteristic, solver);
+        default: return mpick1<5>(characteristic, solver);
+    }
+}
+
+}  // namespace fvm
