@@ -175,6 +175,55 @@ enum {
 int rpfvm_set_kernel_timing(rpfvm_ctx *ctx, int enabled);
 int rpfvm_kernel_times(rpfvm_ctx *ctx, double ms[RPFVM_KCLASS_COUNT], int counts[RPFVM_KCLASS_COUNT]);
 
+/* ---------------------------------------------------------------------------
+ * Batching form: many independent problems of uneven size advanced together
+ * (no counterpart in the reference, whose driver runs one problem per process,
+ * src/RP_FVM.c:92-372).  Each problem keeps its own configuration (grid, dx/dy,
+ * gamma, CFL, Riemann solver, boundary conditions, source) and its own dt and
+ * clock; recon_order (5), characteristic, time_order and precision must be the
+ * same for every problem, and the positivity safeguard is not available.  One RK
+ * stage of all active problems is one persistent launch driven by a work queue
+ * of strips; the whole step is captured once in a HIP graph and replayed.  Each
+ * problem's interior evolves bit-identically to an rpfvm_ctx with the same
+ * configuration driven by rpfvm_advance_to(t_end); ghost layers are not stored
+ * (boundary conditions are applied on load), so rpfvm_batch_get_state returns the
+ * interior and leaves the ghost entries of U as rpfvm_batch_set_state wrote them.
+ * ------------------------------------------------------------------------- */
+typedef struct rpfvm_batch rpfvm_batch;
+
+enum { RPFVM_STAGING_DIRECT = 0, RPFVM_STAGING_LDS = 1 };
+
+/* Launch shape of the stage kernel.  rpfvm_batch_tune measures candidates on the
+ * batch's current state and keeps the fastest. */
+typedef struct rpfvm_batch_tuning {
+    int staging;        /* RPFVM_STAGING_*: stencils from cache or from an LDS row ring */
+    int rows;           /* rows a wavefront marches per strip (4..1024) */
+    int waves_per_cu;   /* resident wavefronts of the persistent grid per CU (1..32) */
+    int use_graph;      /* 1: replay the captured step graph; 0: launch eagerly */
+    double ms_per_stage;/* measured stage time of this shape (0 when set by hand) */
+} rpfvm_batch_tuning;
+
+int rpfvm_batch_create(int count, const rpfvm_config *cfgs, rpfvm_batch **out);
+int rpfvm_batch_destroy(rpfvm_batch *b);
+/* Host U[4][nx+8][ny+8] of problem k; resets its clock to t = 0. */
+int rpfvm_batch_set_state(rpfvm_batch *b, int k, const double *U);
+int rpfvm_batch_get_state(rpfvm_batch *b, int k, double *U);
+/* One step of every problem with t < t_end (dt clipped to reach t_end exactly, as
+ * src/RP_FVM.c:186-191).  Asynchronous. */
+int rpfvm_batch_step(rpfvm_batch *b, double t_end);
+/* Steps until every problem has reached t_end or max_steps steps were launched;
+ * *steps = steps launched.  Synchronous. */
+int rpfvm_batch_advance_to(rpfvm_batch *b, double t_end, int max_steps, int *steps);
+/* Per-problem time, step count and fault bits (arrays of count entries; any may
+ * be NULL).  Synchronous. */
+int rpfvm_batch_status(rpfvm_batch *b, double *t, int *steps, int *faults);
+/* OR of the fault bits of all problems since the last call (cleared). */
+int rpfvm_batch_sync(rpfvm_batch *b);
+int rpfvm_batch_tune(rpfvm_batch *b, rpfvm_batch_tuning *chosen);
+int rpfvm_batch_set_tuning(rpfvm_batch *b, const rpfvm_batch_tuning *t);
+int rpfvm_batch_get_tuning(rpfvm_batch *b, rpfvm_batch_tuning *t);
+int rpfvm_batch_stream(rpfvm_batch *b, void **stream);
+
 /* Message of the last error seen on this thread. */
 const char *rpfvm_last_error(void);
 

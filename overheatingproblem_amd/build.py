@@ -19,6 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "librpfvm.so")
+PROBE = os.path.join(LIBDIR, "hbm_probe")
 BUILD = os.path.join(PKG, "_build")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -29,7 +30,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract
           "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable"]
 
-SOURCES = ["fvm_kernels.hip", "fvm_mixed_kernels.hip", "fvm_api.cpp"]
+SOURCES = ["fvm_kernels.hip", "fvm_mixed_kernels.hip", "fvm_batch_kernels.hip", "fvm_api.cpp",
+           "fvm_batch_api.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -54,8 +56,13 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the copy kernel that calibrates rocprofv3's HBM byte counters
+    cmd = [HIPCC, *COMMON, "-xhip", os.path.join(CSRC, "hbm_probe.cpp"), "-o", PROBE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PROBE}")
     return LIB
 
 

@@ -22,6 +22,8 @@ import numpy as np
 
 from .config import Config
 
+GHOST = 4  # ghost layers per side
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "librpfvm.so")
 
@@ -32,7 +34,21 @@ EXPORTS = (
     "rpfvm_space_operator", "rpfvm_advance_to", "rpfvm_sync", "rpfvm_get_faults",
     "rpfvm_device_state", "rpfvm_stream", "rpfvm_set_kernel_timing", "rpfvm_kernel_times",
     "rpfvm_last_error", "rpfvm_version",
+    "rpfvm_batch_create", "rpfvm_batch_destroy", "rpfvm_batch_set_state", "rpfvm_batch_get_state",
+    "rpfvm_batch_step", "rpfvm_batch_advance_to", "rpfvm_batch_status", "rpfvm_batch_sync",
+    "rpfvm_batch_tune", "rpfvm_batch_set_tuning", "rpfvm_batch_get_tuning", "rpfvm_batch_stream",
 )
+
+STAGING_DIRECT, STAGING_LDS = 0, 1
+
+
+class BatchTuning(ctypes.Structure):
+    """rpfvm_batch_tuning (include/rpfvm.h)."""
+    _fields_ = [("staging", ctypes.c_int), ("rows", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("ms_per_stage", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 KERNEL_CLASSES = ("bc", "flux_x", "flux_y", "update", "dt")
 
@@ -71,6 +87,19 @@ def lib():
     L.rpfvm_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.rpfvm_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.rpfvm_kernel_times.argtypes = [vp, dp, ip]
+    tp = ctypes.POINTER(BatchTuning)
+    L.rpfvm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.rpfvm_batch_destroy.argtypes = [vp]
+    L.rpfvm_batch_set_state.argtypes = [vp, ctypes.c_int, dp]
+    L.rpfvm_batch_get_state.argtypes = [vp, ctypes.c_int, dp]
+    L.rpfvm_batch_step.argtypes = [vp, ctypes.c_double]
+    L.rpfvm_batch_advance_to.argtypes = [vp, ctypes.c_double, ctypes.c_int, ip]
+    L.rpfvm_batch_status.argtypes = [vp, dp, ip, ip]
+    L.rpfvm_batch_sync.argtypes = [vp]
+    L.rpfvm_batch_tune.argtypes = [vp, tp]
+    L.rpfvm_batch_set_tuning.argtypes = [vp, tp]
+    L.rpfvm_batch_get_tuning.argtypes = [vp, tp]
+    L.rpfvm_batch_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.rpfvm_last_error.restype = ctypes.c_char_p
     L.rpfvm_version.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -186,6 +215,99 @@ class Solver:
     def stream(self) -> int:
         s = ctypes.c_void_p()
         _check(lib().rpfvm_stream(self._ctx, ctypes.byref(s)))
+        return s.value or 0
+
+
+class Batch:
+    """Many independent problems advanced together (rpfvm_batch_*).  Each problem has
+    its own Config; recon_order 5, characteristic, time_order and precision are shared.
+    """
+
+    def __init__(self, cfgs):
+        L = lib()
+        self.cfgs = [c.copy() for c in cfgs]
+        arr = (Config * len(self.cfgs))(*self.cfgs)
+        self._h = ctypes.c_void_p()
+        _check(L.rpfvm_batch_create(len(self.cfgs), arr, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rpfvm_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def shape(self, k: int):
+        c = self.cfgs[k]
+        return (4, c.nx + 2 * GHOST, c.ny + 2 * GHOST)
+
+    def set_state(self, k: int, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.shape(k):
+            raise ValueError(f"state of problem {k} must have shape {self.shape(k)}, got {U.shape}")
+        _check(lib().rpfvm_batch_set_state(self._h, k, _dptr(U)))
+
+    def get_state(self, k: int, out: np.ndarray | None = None) -> np.ndarray:
+        """Problem k's state; the interior is current, ghost entries are those of `out`
+        (zeros when not given)."""
+        U = np.zeros(self.shape(k)) if out is None else out
+        _check(lib().rpfvm_batch_get_state(self._h, k, _dptr(U)))
+        return U
+
+    def step(self, t_end: float) -> None:
+        _check(lib().rpfvm_batch_step(self._h, float(t_end)))
+
+    def advance_to(self, t_end: float, max_steps: int = 1 << 30) -> int:
+        n = ctypes.c_int()
+        _check(lib().rpfvm_batch_advance_to(self._h, float(t_end), int(max_steps), ctypes.byref(n)))
+        return n.value
+
+    def status(self):
+        """(t, steps, faults) arrays, one entry per problem."""
+        n = len(self.cfgs)
+        t = np.zeros(n)
+        st = np.zeros(n, dtype=np.int32)
+        f = np.zeros(n, dtype=np.int32)
+        _check(lib().rpfvm_batch_status(self._h, _dptr(t), st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        f.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return t, st, f
+
+    def sync(self) -> int:
+        return _check(lib().rpfvm_batch_sync(self._h))
+
+    def tune(self) -> dict:
+        t = BatchTuning()
+        _check(lib().rpfvm_batch_tune(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def get_tuning(self) -> dict:
+        t = BatchTuning()
+        _check(lib().rpfvm_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw) -> None:
+        t = BatchTuning()
+        _check(lib().rpfvm_batch_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().rpfvm_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(lib().rpfvm_batch_stream(self._h, ctypes.byref(s)))
         return s.value or 0
 
 

@@ -196,3 +196,16 @@ def batch_problems(count=64, seed=20240611, sizes=range(256, 1537, 64)):
         scheme = (2, 1, 3)[int(rng.integers(0, 3))]
         out.append((n, states, scheme))
     return out
+
+
+def batch_case(count=64, seed=20240611, sizes=range(256, 1537, 64), *, characteristic=True,
+               cfl=0.5, **cfg_kw):
+    """Configs and initial states of :func:`batch_problems` (outflow on every side):
+    returns (cfgs, states)."""
+    cfgs, states = [], []
+    for n, quad, scheme in batch_problems(count, seed, sizes):
+        c, U = four_quadrant(n, n, quad, scheme=scheme, characteristic=characteristic, cfl=cfl,
+                             **cfg_kw)
+        cfgs.append(c)
+        states.append(U)
+    return cfgs, states

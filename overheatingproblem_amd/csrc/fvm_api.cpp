@@ -14,13 +14,17 @@
 #include <vector>
 
 #include "../../include/rpfvm.h"
+#include "fvm_api_util.hpp"
 #include "fvm_kernels.hpp"
 
 using namespace fvm;
 
-namespace {
+namespace fvm {
+namespace api {
 
+namespace {
 thread_local char g_err[512] = "";
+}
 
 int fail(int code, const char* fmt, ...) {
     va_list ap;
@@ -30,16 +34,13 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHK(expr)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(RPFVM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+const char* last_error() { return g_err; }
 
-constexpr int kFrontPad = 12;  // makes column GC of every row 128-byte aligned
+}  // namespace api
+}  // namespace fvm
 
-}  // namespace
+using fvm::api::fail;
+using fvm::api::kFrontPad;
 
 struct rpfvm_ctx {
     rpfvm_config cfg;
@@ -60,6 +61,7 @@ struct rpfvm_ctx {
     unsigned* faults = nullptr;
     double* host_scal = nullptr;   // pinned mirror of scal
     unsigned* host_faults = nullptr;
+    unsigned host_pending_faults = 0;  // fault bits found on the host (advance_to)
     // kernel timing (rpfvm_set_kernel_timing)
     bool timing = false;
     struct Pending {
@@ -104,9 +106,30 @@ struct Timed {
 
 }  // namespace
 
+// Checks one step configuration (shared with the batch entry points).
+int fvm::api::validate(const rpfvm_config* c) {
+    if (!c) return fail(RPFVM_ERR_INVALID, "null config");
+    if (c->nx < 1 || c->ny < 1) return fail(RPFVM_ERR_INVALID, "nx, ny must be >= 1");
+    if ((long)(c->nx + 2 * GC) * (c->ny + 2 * GC) > (1L << 31))
+        return fail(RPFVM_ERR_INVALID, "grid too large");
+    if (c->time_order < 1 || c->time_order > 3)
+        return fail(RPFVM_ERR_INVALID, "time_order must be 1, 2 or 3");
+    if (c->recon_order != 1 && c->recon_order != 2 && c->recon_order != 3 && c->recon_order != 5)
+        return fail(RPFVM_ERR_INVALID, "recon_order must be 1, 2, 3 or 5");
+    if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
+        return fail(RPFVM_ERR_UNSUPPORTED, "scheme %d is not implemented on the device", c->scheme);
+    if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
+        return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
+    for (int s = 0; s < 4; s++)
+        if (c->bc[s] < 0 || c->bc[s] > 4) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
+    if (!(c->dx > 0) || !(c->dy > 0) || !(c->gamma > 1.0))
+        return fail(RPFVM_ERR_INVALID, "dx, dy must be > 0 and gamma > 1");
+    return 0;
+}
+
 extern "C" {
 
-const char* rpfvm_last_error(void) { return g_err; }
+const char* rpfvm_last_error(void) { return fvm::api::last_error(); }
 
 const char* rpfvm_version(void) { return "rpfvm-gfx950 0.1.0"; }
 
@@ -130,30 +153,10 @@ void rpfvm_config_default(rpfvm_config* c) {
     }
 }
 
-static int validate(const rpfvm_config* c) {
-    if (!c) return fail(RPFVM_ERR_INVALID, "null config");
-    if (c->nx < 1 || c->ny < 1) return fail(RPFVM_ERR_INVALID, "nx, ny must be >= 1");
-    if ((long)(c->nx + 2 * GC) * (c->ny + 2 * GC) > (1L << 31))
-        return fail(RPFVM_ERR_INVALID, "grid too large");
-    if (c->time_order < 1 || c->time_order > 3)
-        return fail(RPFVM_ERR_INVALID, "time_order must be 1, 2 or 3");
-    if (c->recon_order != 1 && c->recon_order != 2 && c->recon_order != 3 && c->recon_order != 5)
-        return fail(RPFVM_ERR_INVALID, "recon_order must be 1, 2, 3 or 5");
-    if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
-        return fail(RPFVM_ERR_UNSUPPORTED, "scheme %d is not implemented on the device", c->scheme);
-    if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
-        return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
-    for (int s = 0; s < 4; s++)
-        if (c->bc[s] < 0 || c->bc[s] > 4) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
-    if (!(c->dx > 0) || !(c->dy > 0) || !(c->gamma > 1.0))
-        return fail(RPFVM_ERR_INVALID, "dx, dy must be > 0 and gamma > 1");
-    return 0;
-}
-
 int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     if (!out) return fail(RPFVM_ERR_INVALID, "null output pointer");
     *out = nullptr;
-    int e = validate(cfg);
+    int e = fvm::api::validate(cfg);
     if (e) return e;
     rpfvm_ctx* x = new (std::nothrow) rpfvm_ctx();
     if (!x) return fail(RPFVM_ERR_NOMEM, "out of host memory");
@@ -422,6 +425,13 @@ int rpfvm_advance_to(rpfvm_ctx* x, double t_stop, int max_steps, double* t, int*
         if (e) return e;
         if (time + dt > t_stop) dt = t_stop - time;
         if (time + dt >= t_stop) dt = t_stop - time;
+        // The reference loop spins forever when dt stops advancing the clock (a
+        // blown-up state with huge wave speeds); here that ends the loop with the
+        // non-finite-dt fault bit instead.
+        if (!(time + dt > time)) {
+            x->host_pending_faults |= RPFVM_FAULT_NONFINITE_DT;
+            break;
+        }
         if ((e = enqueue_step(x, dt))) return e;
         time = time + dt;
         n++;
@@ -437,7 +447,8 @@ int rpfvm_get_faults(rpfvm_ctx* x, int* faults) {
     HIPCHK(hipMemcpyAsync(x->host_faults, x->faults, sizeof(unsigned), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipMemsetAsync(x->faults, 0, sizeof(unsigned), x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    *faults = (int)*x->host_faults;
+    *faults = (int)(*x->host_faults | x->host_pending_faults);
+    x->host_pending_faults = 0;
     return 0;
 }
 

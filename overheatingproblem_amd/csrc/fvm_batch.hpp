@@ -1,0 +1,107 @@
+// fvm_batch.hpp -- many independent problems of uneven size, advanced together.
+//
+// Every problem keeps its own grid, physics, Riemann solver, boundary conditions, dt
+// and clock; reconstruction, time order and precision are shared by the batch.  One
+// RK stage of every active problem is ONE persistent launch (k_batch_stage) whose
+// wavefronts pull work items ("strips") from a device counter:
+//
+//   strip = (problem, TX consecutive rows, 63 consecutive columns)
+//
+// A wavefront marches down its strip one row at a time: lane t owns the y face
+// G[i][jf] with jf = j0 - 1 + t and the cell (i, jf + 1); the x-face flux F[i][c] of the
+// row is carried in registers to the next row as F[i-1][c], and the cell's upper y face
+// comes from lane t + 1 by shuffle.  Face fluxes therefore never reach HBM: per stage
+// the kernel reads the stage input (plus U^n) and writes the stage output.  Ghost
+// cells are not stored: a halo load outside the interior goes through the side's
+// boundary map (the index map and sign of Boundary_Conditions), so there is no
+// ghost-fill pass either.
+//
+// The face and cell arithmetic is the single-problem path's (fvm_math.hpp,
+// fvm_mixed.hpp), so each problem's interior is bit-identical to advancing it alone.
+// The per-problem CFL reduction of the next step is folded into the last stage
+// (atomic max of bit patterns of non-negative doubles, which order as unsigned
+// integers), and k_batch_control turns it into each problem's dt, clock, activity and
+// the schedule of active strips before stage 1.  The step is a fixed sequence of
+// launches on one stream, captured once into a hipGraph and replayed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "fvm_kernels.hpp"
+
+namespace fvm {
+
+constexpr int kStripCols = 63;  // cells per strip row (64 y faces)
+
+enum : int { STAGING_DIRECT = 0, STAGING_LDS = 1 };
+
+// One problem of the batch (device-resident, read with scalar loads).
+struct BProb {
+    Geom g;
+    Phys ph;
+    int nx, ny;
+    int solver;
+    int bc[4];            // L, R, B, T
+    double bcv[4][4];     // inflow / fixed state per side
+    size_t off;           // offset in doubles of the problem's planes in each pool buffer
+    int strips_y;         // ceil(ny / 63)
+    int pad;
+};
+
+// Per-problem state of the time loop.
+struct BStat {
+    double t;                      // time reached after the step in flight
+    double dt;                     // dt of the step in flight
+    unsigned long long smax[2];    // bits of max(|u|+a), max(|v|+a) over the interior
+    int steps;                     // steps taken
+    int active;                    // taking part in the step in flight
+    unsigned faults;               // RPFVM_FAULT_* bits
+    int pad;
+};
+
+// Schedule of the step in flight, written by k_batch_control.
+struct BSched {
+    unsigned queue[4];    // work counters of stages 1..3 (+1 spare)
+    unsigned defer[4];    // deferred-cell counts of stages 1..3 (mixed form)
+    int nact;             // active problems
+    int total;            // active strips
+    int active_left;      // problems with t < t_end after this step's dt
+    int pad;
+    double t_end;         // stop time (set by the host)
+};
+
+struct BatchArgs {
+    const BProb* probs;
+    BStat* stat;
+    BSched* sched;
+    int* act_prob;             // [nact] problem index of each active problem
+    int* act_start;            // [nact + 1] first strip of each active problem
+    double* pool[3];           // U, U1, U2 of every problem (problem k at + probs[k].off)
+    unsigned long long* defer_list;  // (problem << 40) | (i << 20) | j of deferred cells
+    unsigned* faults;          // batch-wide OR of fault bits
+    int nprob;
+    int rows;                  // TX: rows per strip
+};
+
+struct BatchStageLaunch {
+    int staging;      // STAGING_*
+    int grid;         // persistent workgroups
+    int block;        // threads per workgroup (multiple of 64)
+};
+
+// Launchers (fvm_batch_kernels.hip).  `stage` is 0-based; the kinds and buffers of
+// each stage follow time_order (1: U -> U1 (+copy back), 2: U -> U1 -> U,
+// 3: U -> U1 -> U2 -> U).
+struct BatchKernels {
+    void (*stage)(const BatchArgs& a, int time_order, int stage, const BatchStageLaunch& l,
+                  hipStream_t s);
+};
+BatchKernels select_batch(int order, bool characteristic, bool mixed);
+void launch_batch_control(const BatchArgs& a, int mode, hipStream_t s);
+void launch_batch_copy_back(const BatchArgs& a, hipStream_t s);
+void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s);
+void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s);
+
+}  // namespace fvm

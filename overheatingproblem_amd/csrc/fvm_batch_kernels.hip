@@ -1,0 +1,607 @@
+// fvm_batch_kernels.hip -- kernels of the batching form (layout and schedule in
+// fvm_batch.hpp).
+//
+//   k_batch_control   one workgroup: per-problem dt / clock / activity from the
+//                     previous step's wave-speed maxima (Get_Delta_T_2D,
+//                     include/function.h:144-175, clipped as src/RP_FVM.c:186-191), the
+//                     prefix table of active strips, and the reset of the work counters
+//   k_batch_stage     persistent: one RK stage of every active problem, strip by strip
+//   k_batch_fix       mixed form: the cells next to faces whose FP32 states were not
+//                     admissible, with those faces recomputed in FP64
+//   k_batch_copy_back RK1 only: U1 -> U over the active interiors
+#include <hip/hip_runtime.h>
+
+#include "fvm_batch.hpp"
+#include "fvm_kernels.hpp"
+#include "fvm_math.hpp"
+#include "fvm_mixed.hpp"
+
+namespace fvm {
+namespace {
+
+// Stage input value at (i, j) of one problem with the ghost layers replaced by the
+// boundary maps of Boundary_Conditions (include/Boundary_Condition.h:63-465, as
+// k_bc_side applies them): outflow copies the first interior cell, reflection mirrors
+// on the left and top and copies the first interior cell on the right and bottom, with
+// the normal momentum negated, periodic wraps by n, inflow / fixed return the side's
+// state.  At most one of i, j lies outside the interior.
+__device__ __forceinline__ double bload(const double* P, const BProb& d, int i, int j, int v) {
+    const Geom& g = d.g;
+    bool neg = false;
+    if (i < GC || i >= g.R - GC) {
+        const int s = i < GC ? 0 : 1;
+        switch (d.bc[s]) {
+            case BC_OUT: i = s == 0 ? GC : g.R - GC - 1; break;
+            case BC_REFLECT:
+                i = s == 0 ? 2 * GC - 1 - i : g.R - GC - 1;
+                neg = v == 1;
+                break;
+            case BC_PERIODIC: i = s == 0 ? i + d.nx : i - d.nx; break;
+            default: return d.bcv[s][v];
+        }
+    } else if (j < GC || j >= g.C - GC) {
+        const int s = j < GC ? 2 : 3;
+        switch (d.bc[s]) {
+            case BC_OUT: j = s == 2 ? GC : g.C - GC - 1; break;
+            case BC_REFLECT:
+                j = s == 2 ? GC : 2 * (g.C - GC) - 1 - j;
+                neg = v == 2;
+                break;
+            case BC_PERIODIC: j = s == 2 ? j + d.ny : j - d.ny; break;
+            default: return d.bcv[s][v];
+        }
+    }
+    const double x = P[(size_t)v * g.plane + (size_t)i * g.ld + j];
+    return neg ? -x : x;
+}
+
+// FP64 face (the single-problem k_flux arithmetic) with the solver chosen at run time.
+template <int DIR, int ORDER, bool CHAR>
+__device__ __forceinline__ int face64(const Consts& kc, const Phys& ph, int solver,
+                                      const double st[6][4], double f[4]) {
+    double UL[4], UR[4];
+    reconstruct_face<DIR, ORDER, CHAR>(kc, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    switch (solver) {
+        case SOLVER_HLL: return riemann<DIR, SOLVER_HLL>(kc, UL, UR, f);
+        case SOLVER_HLLC: return riemann<DIR, SOLVER_HLLC>(kc, UL, UR, f);
+        case SOLVER_ROE: return riemann<DIR, SOLVER_ROE>(kc, UL, UR, f);
+        default: return riemann<DIR, SOLVER_LLF>(kc, UL, UR, f);
+    }
+}
+
+// Mixed-form face (k_flux_mixed arithmetic); false when the FP32-reconstructed states
+// are not admissible and the face must be recomputed in FP64.
+template <int DIR, int ORDER, bool CHAR>
+__device__ __forceinline__ bool face_mx(const Phys& ph, int solver, const double st[6][4],
+                                        double f[4]) {
+    double UL[4], UR[4];
+    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
+        return false;
+    switch (solver) {
+        case SOLVER_HLL: mx::riemann<DIR, SOLVER_HLL>(ph.gamma, UL, UR, f); break;
+        case SOLVER_HLLC: mx::riemann<DIR, SOLVER_HLLC>(ph.gamma, UL, UR, f); break;
+        case SOLVER_ROE: mx::riemann<DIR, SOLVER_ROE>(ph.gamma, UL, UR, f); break;
+        default: mx::riemann<DIR, SOLVER_LLF>(ph.gamma, UL, UR, f); break;
+    }
+    return true;
+}
+
+// Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751) and
+// the RK stage combine (include/time_advance.h:697, 757-777, 832-872) of one cell, with
+// the four face fluxes in registers: fl/fu the lower/upper x faces, gl/gu the y faces.
+// The expressions are cell_update's, term for term.
+__device__ __forceinline__ void combine(const Phys& ph, double dt, int kind, const double y[4],
+                                        const double un[4], const double fl[4],
+                                        const double fu[4], const double gl[4],
+                                        const double gu[4], double r[4]) {
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ph.source) {
+        const double rho = y[0];
+        const double v = y[2] / rho;
+        S[2] = rho * ph.gravity;
+        S[3] = rho * ph.gravity * v;
+    }
+    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double L = -(fu[k] - fl[k]) * idx - (gu[k] - gl[k]) * idy + S[k];
+        switch (kind) {
+            case UPD_EULER: r[k] = y[k] + dt * L; break;
+            case UPD_RK2_2: r[k] = 0.5 * un[k] + 0.5 * (y[k] + dt * L); break;
+            case UPD_RK3_2: r[k] = (3.0 / 4.0) * un[k] + (1.0 / 4.0) * (y[k] + dt * L); break;
+            default: r[k] = (1.0 / 3.0) * un[k] + (2.0 / 3.0) * (y[k] + dt * L); break;
+        }
+    }
+}
+
+// |u| + a and |v| + a of a cell, as k_dt_partial.
+__device__ __forceinline__ void speeds(double gam, const double c[4], double& lx, double& ly) {
+    const double rho = c[0];
+    const double u = c[1] / rho;
+    const double v = c[2] / rho;
+    const double p = (c[3] - 0.5 * rho * (u * u + v * v)) * (gam - 1.0);
+    const double a = sqrt(gam * p / rho);
+    lx = fabs(u) + a;
+    ly = fabs(v) + a;
+}
+
+__device__ __forceinline__ double nanmax_b(double a, double b) { return b > a ? b : a; }
+
+__device__ __forceinline__ double wave_max_b(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = nanmax_b(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+// Non-negative doubles order as their bit patterns.
+__device__ __forceinline__ void speed_max(BStat& s, double sx, double sy) {
+    atomicMax(&s.smax[0], (unsigned long long)__double_as_longlong(sx));
+    atomicMax(&s.smax[1], (unsigned long long)__double_as_longlong(sy));
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// A wave-uniform row index the optimiser cannot derive from the loop counter.
+__device__ __forceinline__ int opaque_row(int r) {
+    asm volatile("" : "+s"(r));
+    return r;
+}
+
+// Which active problem owns strip `item` (binary search of the prefix table).
+__device__ __forceinline__ int owner(const BatchArgs& a, int nact, int item) {
+    int lo = 0, hi = nact - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.act_start[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+constexpr int kRingCols = 72;  // 69 columns of a strip row (+3 padding)
+
+}  // namespace
+
+// ----------------------------------------------------------------- control
+// mode 0: advance the clocks (one step); mode 1: schedule only (autotuner trials).
+// A problem stops at t_end or on a dt fault; an HLL bad-state fault (bit 1) does not
+// stop it, as the reference's HLL_Flux prints and carries on (include/scheme.h:427-436).
+__global__ void k_batch_control(BatchArgs a, int mode) {
+    BSched& sc = *a.sched;
+    const double t_end = sc.t_end;
+    for (int k = threadIdx.x; k < a.nprob; k += blockDim.x) {
+        BStat& s = a.stat[k];
+        const BProb& d = a.probs[k];
+        if (mode == 0) {
+            bool act = s.t < t_end && !(s.faults & 2u);
+            if (act) {
+                const double sx = __longlong_as_double((long long)s.smax[0]);
+                const double sy = __longlong_as_double((long long)s.smax[1]);
+                double dt = d.ph.cfl * min2(d.ph.dx, d.ph.dy) / (sx + sy);
+                if (s.t + dt > t_end) dt = t_end - s.t;
+                if (s.t + dt >= t_end) dt = t_end - s.t;
+                // a dt that no longer advances the clock is a fault too (the loop
+                // would never reach t_end)
+                if (!(dt > 0.0) || !isfinite(dt) || !(s.t + dt > s.t)) {
+                    s.faults |= 2u;
+                    atomicOr(a.faults, 2u);
+                    act = false;
+                } else {
+                    s.dt = dt;
+                    s.t = s.t + dt;
+                    s.steps += 1;
+                    s.smax[0] = 0ull;
+                    s.smax[1] = 0ull;
+                }
+            }
+            s.active = act ? 1 : 0;
+        } else {
+            s.active = (s.t < t_end && !(s.faults & 2u)) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int nact = 0, total = 0, left = 0;
+        bool any = false;
+        for (int k = 0; k < a.nprob; k++) any = any || a.stat[k].active;
+        for (int k = 0; k < a.nprob; k++) {
+            const BStat& s = a.stat[k];
+            const bool take = mode == 0 ? s.active != 0 : (s.active != 0 || !any);
+            if (s.t < t_end && !(s.faults & 2u)) left++;
+            if (!take) continue;
+            const BProb& d = a.probs[k];
+            a.act_prob[nact] = k;
+            a.act_start[nact] = total;
+            total += ((d.nx + a.rows - 1) / a.rows) * d.strips_y;
+            nact++;
+        }
+        a.act_start[nact] = total;
+        sc.nact = nact;
+        sc.total = total;
+        sc.active_left = left;
+        for (int q = 0; q < 4; q++) {
+            sc.queue[q] = 0u;
+            sc.defer[q] = 0u;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- stage
+// One RK stage: Y = pool[yi] -> out = pool[oi] (U^n = pool[0]); `last` accumulates the
+// wave-speed maxima of the new state for the next step's dt.
+template <int ORDER, bool CHAR, bool MIXED, int STAGING>
+__global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int kind, int yi,
+                                                    int oi, int last) {
+    const int lane = threadIdx.x;
+    const int nact = a.sched->nact, total = a.sched->total;
+    unsigned* queue = &a.sched->queue[stage];
+    __shared__ double ring[STAGING == STAGING_LDS ? 6 : 1][4][kRingCols];
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(queue, 1u);
+        item = uniform(__shfl(item, 0, 64));
+        if (item >= total) break;
+        const int ai = uniform(owner(a, nact, item));
+        const int k = uniform(a.act_prob[ai]);
+        const int local = item - a.act_start[ai];
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const Phys ph = d.ph;
+        const int solver = d.solver;
+        const double dt = a.stat[k].dt;
+        const double* Y = a.pool[yi] + d.off;
+        const double* Un = a.pool[0] + d.off;
+        double* out = a.pool[oi] + d.off;
+        const int sx = local / d.strips_y, sy = local - sx * d.strips_y;
+        const int i0 = GC + sx * a.rows;
+        const int i1 = min(i0 + a.rows, g.R - GC);
+        const int j0 = GC + sy * kStripCols;
+        const int jf = j0 - 1 + lane;                 // this lane's y face G[i][jf]
+        const int c = jf + 1;                         // this lane's cell column
+        const bool face_ok = jf <= g.C - GC - 1;
+        const bool cell_ok = lane < kStripCols && c <= g.C - GC - 1;
+        int fault = 0;
+        double smx = 0.0, smy = 0.0;
+
+        // x-face stencil rows r-2..r+3 at column c, y-face stencil columns jf-2..jf+3
+        // at row r
+        // Row indices pass through opaque_row before addressing: left visible, loop
+        // strength reduction turns the 24 stencil addresses into 24 live 64-bit
+        // induction pointers (about 50 VGPRs).
+        auto load_x = [&](int r, double st[6][4]) {
+            r = opaque_row(r);
+            if constexpr (STAGING == STAGING_LDS) {
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = ring[(r - 2 + n + 6) % 6][v][lane + 3];
+            } else {
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = bload(Y, d, r - 2 + n, c, v);
+            }
+        };
+        auto load_y = [&](int r, double st[6][4]) {
+            r = opaque_row(r);
+            if constexpr (STAGING == STAGING_LDS) {
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = ring[r % 6][v][lane + n];
+            } else {
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = bload(Y, d, r, jf - 2 + n, v);
+            }
+        };
+        // LDS ring: row r, columns j0-3 .. j0+65 (clamped into the array) at slot r % 6
+        auto stage_row = [&](int r) {
+            if constexpr (STAGING == STAGING_LDS) {
+                r = opaque_row(r);
+                const int ja = min(j0 - 3 + lane, g.C - 1);
+                const int jb = min(j0 + 61 + lane, g.C - 1);
+                const bool interior_row = r >= GC && r < g.R - GC;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    // a ghost row is read only at interior columns; a ghost column only
+                    // in interior rows
+                    const double xa = (interior_row || (ja >= GC && ja < g.C - GC))
+                                          ? bload(Y, d, r, ja, v) : 0.0;
+                    ring[r % 6][v][lane] = xa;
+                    if (lane < 5) {
+                        const double xb = (interior_row || (jb >= GC && jb < g.C - GC))
+                                              ? bload(Y, d, r, jb, v) : 0.0;
+                        ring[r % 6][v][64 + lane] = xb;
+                    }
+                }
+            }
+        };
+        // The face arithmetic reads the problem's physics through a pointer the
+        // compiler cannot see through, once per row: otherwise it hoists every
+        // gamma-derived constant out of the row loop and keeps them all live, which
+        // costs 40-60 VGPRs (and occupancy).
+        auto row_phys = [&]() {
+            const BProb* dq = a.probs + k;
+            asm volatile("" : "+s"(dq));
+            return dq->ph;
+        };
+        auto x_face = [&](int r, double f[4]) -> bool {
+            double st[6][4];
+            load_x(r, st);
+            const Phys pr = row_phys();
+            if constexpr (MIXED) {
+                return face_mx<1, ORDER, CHAR>(pr, solver, st, f);
+            } else {
+                fault |= face64<1, ORDER, CHAR>(make_consts(pr), pr, solver, st, f);
+                return true;
+            }
+        };
+
+        if constexpr (STAGING == STAGING_LDS) {
+            __syncthreads();  // the previous strip's reads are done
+            for (int r = i0 - 3; r < i0 + 2; r++) stage_row(r);
+        }
+        // Row r computes the x face F[r] (between rows r and r + 1); from r = i0 on it
+        // also computes the y faces of row r and updates the row's cells with F[r - 1]
+        // carried from the previous iteration.
+        double fp[4] = {0.0, 0.0, 0.0, 0.0};
+        bool bad_p = false;
+        for (int i = i0 - 1; i < i1; i++) {
+            if constexpr (STAGING == STAGING_LDS) {
+                __syncthreads();     // slot (i + 3) % 6 held row i - 3, last read for face i - 1
+                stage_row(i + 3);
+                __syncthreads();
+            }
+            double fc[4] = {0.0, 0.0, 0.0, 0.0};
+            bool bad_c = false;
+            if (cell_ok) bad_c = !x_face(i, fc);
+            if (i >= i0) {
+                double gf[4] = {0.0, 0.0, 0.0, 0.0}, y[4] = {0.0, 0.0, 0.0, 0.0};
+                int bad_g = 0;
+                if (face_ok) {
+                    double st[6][4];
+                    load_y(i, st);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) y[v] = st[3][v];
+                    const Phys pr = row_phys();
+                    if constexpr (MIXED) {
+                        bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
+                    } else {
+                        fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                    }
+                }
+                double gu[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
+                const int bad_gu = __shfl_down(bad_g, 1, 64);
+                if (cell_ok) {
+                    const size_t o0 = (size_t)opaque_row(i) * g.ld + c;
+                    if (MIXED && (bad_p || bad_c || bad_g || bad_gu)) {
+                        const unsigned q = atomicAdd(&a.sched->defer[stage], 1u);
+                        a.defer_list[q] = ((unsigned long long)k << 40) |
+                                          ((unsigned long long)i << 20) | (unsigned long long)c;
+                    } else {
+                        double un[4] = {0.0, 0.0, 0.0, 0.0}, r[4];
+                        if (kind != UPD_EULER) {
+#pragma unroll
+                            for (int v = 0; v < 4; v++) un[v] = Un[(size_t)v * g.plane + o0];
+                        }
+                        combine(ph, dt, kind, y, un, fp, fc, gf, gu, r);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) out[(size_t)v * g.plane + o0] = r[v];
+                        if (last) {
+                            double lx, ly;
+                            speeds(ph.gamma, r, lx, ly);
+                            if (lx > smx) smx = lx;
+                            if (ly > smy) smy = ly;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) fp[v] = fc[v];
+            bad_p = bad_c;
+        }
+        if (last) {
+            smx = wave_max_b(smx);
+            smy = wave_max_b(smy);
+            if (lane == 0) speed_max(a.stat[k], smx, smy);
+        }
+        if (!MIXED) {
+            const unsigned long long any = __ballot(fault != 0);
+            if (any && lane == 0) {
+                atomicOr(a.faults, 1u);
+                atomicOr(&a.stat[k].faults, 1u);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fix-up
+// Mixed form: each deferred cell with its four faces recomputed (mixed arithmetic,
+// FP64 for a face whose FP32 states are not admissible) -- the single-problem mixed
+// path's k_flux_fix + k_yfix_cells, per cell.
+template <int ORDER, bool CHAR>
+__global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int kind, int yi,
+                                                   int oi, int last) {
+    const unsigned n = a.sched->defer[stage];
+    if (n && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.faults, 4u);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const unsigned long long e = a.defer_list[q];
+        const int k = (int)(e >> 40), i = (int)((e >> 20) & 0xFFFFF), c = (int)(e & 0xFFFFF);
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const Phys ph = d.ph;
+        const double* Y = a.pool[yi] + d.off;
+        const double* Un = a.pool[0] + d.off;
+        double* out = a.pool[oi] + d.off;
+        const Consts kc = make_consts(ph);
+        double fl[4], fu[4], gl[4], gu[4];
+        double* faces[4] = {fl, fu, gl, gu};
+        int fault = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            double st[6][4];
+            if (m < 2) {
+                const int r = i - 1 + m;  // x face between rows r and r + 1
+#pragma unroll
+                for (int s = 0; s < 6; s++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[s][v] = bload(Y, d, r - 2 + s, c, v);
+                if (!face_mx<1, ORDER, CHAR>(ph, d.solver, st, faces[m]))
+                    fault |= face64<1, ORDER, CHAR>(kc, ph, d.solver, st, faces[m]);
+            } else {
+                const int jf = c - 1 + (m - 2);  // y face between columns jf and jf + 1
+#pragma unroll
+                for (int s = 0; s < 6; s++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[s][v] = bload(Y, d, i, jf - 2 + s, v);
+                if (!face_mx<2, ORDER, CHAR>(ph, d.solver, st, faces[m]))
+                    fault |= face64<2, ORDER, CHAR>(kc, ph, d.solver, st, faces[m]);
+            }
+        }
+        const size_t o0 = (size_t)i * g.ld + c;
+        double y[4], un[4] = {0.0, 0.0, 0.0, 0.0}, r[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            y[v] = Y[(size_t)v * g.plane + o0];
+            if (kind != UPD_EULER) un[v] = Un[(size_t)v * g.plane + o0];
+        }
+        combine(ph, a.stat[k].dt, kind, y, un, fl, fu, gl, gu, r);
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[(size_t)v * g.plane + o0] = r[v];
+        if (last) {
+            double lx, ly;
+            speeds(ph.gamma, r, lx, ly);
+            speed_max(a.stat[k], lx > 0.0 ? lx : 0.0, ly > 0.0 ? ly : 0.0);
+        }
+        if (fault) {
+            atomicOr(a.faults, 1u);
+            atomicOr(&a.stat[k].faults, 1u);
+        }
+    }
+}
+
+// ------------------------------------------------------------- RK1, speeds
+__global__ __launch_bounds__(256) void k_batch_copy_back(BatchArgs a) {
+    const int nact = a.sched->nact;
+    for (int ai = blockIdx.y; ai < nact; ai += gridDim.y) {
+        const int k = a.act_prob[ai];
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const long total = (long)d.nx * d.ny;
+        for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+             t += (long)gridDim.x * blockDim.x) {
+            const int i = GC + (int)(t / d.ny), j = GC + (int)(t % d.ny);
+            const size_t o = d.off + (size_t)i * g.ld + j;
+#pragma unroll
+            for (int v = 0; v < 4; v++) a.pool[0][(size_t)v * g.plane + o] = a.pool[1][(size_t)v * g.plane + o];
+        }
+    }
+}
+
+// Clock reset and wave-speed maxima of problem k's current state (after set_state).
+__global__ void k_batch_reset(BatchArgs a, int k) {
+    if (threadIdx.x == 0) {
+        BStat& s = a.stat[k];
+        s.t = 0.0;
+        s.dt = 0.0;
+        s.smax[0] = 0ull;
+        s.smax[1] = 0ull;
+        s.steps = 0;
+        s.active = 0;
+        s.faults = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_speeds(BatchArgs a, int k) {
+    const BProb& d = a.probs[k];
+    const Geom g = d.g;
+    const double* U = a.pool[0] + d.off;
+    double sx = 0.0, sy = 0.0;
+    const long total = (long)d.nx * d.ny;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long)gridDim.x * blockDim.x) {
+        const int i = GC + (int)(t / d.ny), j = GC + (int)(t % d.ny);
+        const size_t o = (size_t)i * g.ld + j;
+        double c[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) c[v] = U[(size_t)v * g.plane + o];
+        double lx, ly;
+        speeds(d.ph.gamma, c, lx, ly);
+        if (lx > sx) sx = lx;
+        if (ly > sy) sy = ly;
+    }
+    sx = wave_max_b(sx);
+    sy = wave_max_b(sy);
+    if ((threadIdx.x & 63) == 0) speed_max(a.stat[k], sx, sy);
+}
+
+// --------------------------------------------------------------- launchers
+namespace {
+
+template <int ORDER, bool CHAR, bool MIXED>
+void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStageLaunch& l,
+                  hipStream_t s) {
+    // (Y, out, kind, last) per stage, include/time_advance.h:643-882
+    int yi = 0, oi = 1, kind = UPD_EULER, last = 0;
+    if (time_order == 1) {
+        last = 1;
+    } else if (time_order == 2) {
+        if (stage == 1) {
+            yi = 1;
+            oi = 0;
+            kind = UPD_RK2_2;
+            last = 1;
+        }
+    } else {
+        if (stage == 1) {
+            yi = 1;
+            oi = 2;
+            kind = UPD_RK3_2;
+        } else if (stage == 2) {
+            yi = 2;
+            oi = 0;
+            kind = UPD_RK3_3;
+            last = 1;
+        }
+    }
+    if (l.staging == STAGING_LDS)
+        hipLaunchKernelGGL((k_batch_stage<ORDER, CHAR, MIXED, STAGING_LDS>), dim3(l.grid), dim3(64),
+                           0, s, a, stage, kind, yi, oi, last);
+    else
+        hipLaunchKernelGGL((k_batch_stage<ORDER, CHAR, MIXED, STAGING_DIRECT>), dim3(l.grid),
+                           dim3(64), 0, s, a, stage, kind, yi, oi, last);
+    if constexpr (MIXED)
+        hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(64), dim3(256), 0, s, a, stage, kind, yi,
+                           oi, last);
+}
+
+}  // namespace
+
+BatchKernels select_batch(int order, bool characteristic, bool mixed) {
+    (void)order;  // the batch form runs fifth-order WENO (checked by the API)
+    if (mixed) return {characteristic ? &stage_launch<5, true, true> : &stage_launch<5, false, true>};
+    return {characteristic ? &stage_launch<5, true, false> : &stage_launch<5, false, false>};
+}
+
+void launch_batch_control(const BatchArgs& a, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_control, dim3(1), dim3(256), 0, s, a, mode);
+}
+
+void launch_batch_copy_back(const BatchArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_copy_back, dim3(64, 64), dim3(256), 0, s, a);
+}
+
+void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a, k);
+}
+
+void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_speeds, dim3(256), dim3(256), 0, s, a, k);
+}
+
+}  // namespace fvm
