@@ -167,6 +167,123 @@ def cpu_baseline(n, steps, cores):
                       f"(dt + 3 stages, IO excluded), {secs:.2f} s"}
 
 
+def run_batch(steps, warmup):
+    """The batching form on the seeded 64-problem four-quadrant workload (sizes 256..1536,
+    HLLC/HLL/Roe per problem, WENO5-char, RK3, CFL 0.5): autotune, then time `steps`
+    steps of the whole batch (every problem is still active: the smallest needs ~400
+    steps to reach t = 0.25)."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.capi import Batch
+    ev = HipEvents()
+    cfgs, states = cases.batch_case(64)
+    cells = sum(c.nx * c.ny for c in cfgs)
+    with Batch(cfgs) as b:
+        for k, U in enumerate(states):
+            b.set_state(k, U)
+        tuned = b.tune()
+        for _ in range(warmup):
+            b.step(0.25)
+        b.sync()
+        s = b.stream()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, s)
+        for _ in range(steps):
+            b.step(0.25)
+        ev.record(e1, s)
+        ms = ev.elapsed_ms(e0, e1)
+        t, st, f = b.status()
+    active = int((st == warmup + steps).sum())
+    value = cells * steps / (ms / 1e3) / 1e6
+    return {"value": round(value, 3), "unit": "Mcell-updates/s", "ms_per_step": round(ms / steps, 4),
+            "workload": "64 four-quadrant problems, n in {256..1536}, seed 20240611",
+            "cells": cells, "form": "fp64", "tuning": tuned,
+            "problems_active_throughout": active, "fault_bits": sorted(set(int(x) for x in f)),
+            "note": "the mixed-precision batch form is switched off"}
+
+
+def run_dmr(steps, warmup):
+    """Double Mach reflection 8192 x 2048 (moving boundaries, WENO5-char + Roe with the
+    HLL(E) positivity fallback, CFL 0.5): FP64 and mixed throughput, and the mixed form's
+    L1 / L-inf of rho and p against FP64 (mixed replayed on the FP64 run's dts)."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.config import PRECISION_MIXED
+    cfg, U0 = cases.dmr(8192, 2048)
+    mx = cfg.copy()
+    mx.precision = PRECISION_MIXED
+    cells = cfg.nx * cfg.ny
+    r64 = run_gpu(cfg, U0, steps, warmup)
+    rmx = run_gpu(mx, U0, steps, warmup)
+    rep = run_gpu(mx, U0, steps, warmup, dts=r64["dts"])
+    p64 = cases.cons_to_prim(r64["U"][:, 4:-4, 4:-4], cfg.gamma)
+    pmx = cases.cons_to_prim(rep["U"][:, 4:-4, 4:-4], cfg.gamma)
+
+    def rate(r):
+        return round(cells * steps / (r["gpu_ms"] / 1e3) / 1e6, 3)
+
+    return {"workload": "dmr-8192x2048", "unit": "Mcell-updates/s", "steps": steps,
+            "fp64": rate(r64), "mixed": rate(rmx), "speedup": round(rate(rmx) / rate(r64), 4),
+            "l1_rho": float(np.abs(pmx[0] - p64[0]).mean()), "linf_rho": float(np.abs(pmx[0] - p64[0]).max()),
+            "l1_p": float(np.abs(pmx[3] - p64[3]).mean()), "linf_p": float(np.abs(pmx[3] - p64[3]).max()),
+            "faults": {"fp64": r64["faults"], "mixed": rmx["faults"]}}
+
+
+def noh_metrics(states, n, gamma, t_end=0.6):
+    """Accuracy figures of Noh states {"fp64": U, "mixed": U} (full arrays with ghosts):
+    the wall-heating density deficit 1 - rho/16 at the corner cell and over r < 0.05, the
+    post-shock plateau 0.1 < r < 0.15, and mixed-vs-FP64 L1 / L-inf of rho and p."""
+    from overheatingproblem_amd import cases
+    X, Y = cases.cell_centres(n, n, 1.0 / n, 1.0 / n, ghosts=False)
+    r = np.sqrt(X * X + Y * Y)
+    out = {}
+    prim = {k: cases.cons_to_prim(U[:, 4:-4, 4:-4], gamma) for k, U in states.items()}
+    for k, (rho, u, v, p) in prim.items():
+        out[k] = {"rho_corner": float(rho[0, 0]),
+                  "deficit_corner": float(1.0 - rho[0, 0] / 16.0),
+                  "deficit_r_lt_0.05": float(1.0 - rho[r < 0.05].mean() / 16.0),
+                  "rho_plateau_0.1_0.15": float(rho[(r > 0.1) & (r < 0.15)].mean()),
+                  "shock_radius_exact": t_end / 3.0}
+    if "fp64" in prim and "mixed" in prim:
+        (r64, _, _, p64), (rmx, _, _, pmx) = prim["fp64"], prim["mixed"]
+        out["mixed_vs_fp64"] = {
+            "l1_rho": float(np.abs(rmx - r64).mean()), "linf_rho": float(np.abs(rmx - r64).max()),
+            "l1_rho_rel": float(np.abs(rmx - r64).mean() / np.abs(r64).mean()),
+            "l1_p": float(np.abs(pmx - p64).mean()), "linf_p": float(np.abs(pmx - p64).max()),
+            "l1_p_rel": float(np.abs(pmx - p64).mean() / np.abs(p64).mean())}
+    return out
+
+
+def noh_full(n, t_end=0.6):
+    """Both forms to t_end on the Noh workload: L1 / L-inf of rho and p (mixed vs FP64,
+    interior) and the wall-heating density deficit near the origin.  The exact 2-D Noh
+    solution has rho = 16 behind the shock (r < t/3) for gamma = 5/3, so the deficit is
+    1 - rho/16 at the corner cell and over r < 0.05, with the plateau 0.1 < r < 0.15 as
+    the shocked-state reference."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.capi import Solver
+    from overheatingproblem_amd.config import PRECISION_MIXED
+    cfg, U0 = cases.noh(n, positivity=True)
+    res = {}
+    for name in ("fp64", "mixed"):
+        c = cfg.copy()
+        if name == "mixed":
+            c.precision = PRECISION_MIXED
+        t0 = time.perf_counter()
+        with Solver(c) as s:
+            s.set_state(U0)
+            t, steps = s.advance_to(t_end, 10 ** 7)
+            U = s.get_state()
+            faults = s.sync()
+        res[name] = {"U": U, "t": t, "steps": steps, "faults": faults,
+                     "wall_s": time.perf_counter() - t0}
+        log(f"bench: noh {n}^2 {name} to t={t} in {steps} steps, {res[name]['wall_s']:.1f} s")
+    out = {"workload": f"noh-{n}x{n}", "t_end": t_end}
+    out.update(noh_metrics({k: v["U"] for k, v in res.items()}, n, cfg.gamma))
+    for k, v in res.items():
+        out[k].update({"steps": v["steps"], "faults": v["faults"], "wall_s": round(v["wall_s"], 2),
+                       "t": v["t"]})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,12 +292,19 @@ def main():
     ap.add_argument("--n", type=int, default=4096, help="grid cells per side")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-dmr", action="store_true")
+    ap.add_argument("--noh-full", action="store_true",
+                    help="run both forms to t = 0.6 and print the accuracy report instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has "
               f"no multi-GPU path; see DESIGN.md for how the step would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.noh_full:
+        print(json.dumps(noh_full(args.n)), flush=True)
+        return
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.config import PRECISION_MIXED
     n = args.n
@@ -267,6 +391,18 @@ def main():
                      "note": f"mixed replayed on the fp64 run's {len(r64['dts'])} dts"},
         "wall_s": round(rmx["wall_s"], 4),
     }
+    if not args.no_dmr:
+        log("bench: double Mach reflection 8192x2048 ...")
+        try:
+            out["dmr"] = run_dmr(10, 2)
+        except Exception as e:  # the headline figure stands without it
+            out["dmr"] = {"value": None, "error": str(e)}
+    if not args.no_batch:
+        log("bench: batch form (64 uneven problems) ...")
+        try:
+            out["batch"] = run_batch(10, 2)
+        except Exception as e:  # the headline figure stands without it
+            out["batch"] = {"value": None, "error": str(e)}
     if not args.no_cpu:
         log(f"bench: cpu baseline ({args.cpu_steps} steps at {n}^2) ...")
         try:

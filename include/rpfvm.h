@@ -37,7 +37,17 @@ enum {
     RPFVM_BC_REFLECTION = 1,
     RPFVM_BC_PERIODICITY = 2,
     RPFVM_BC_INFLOW = 3,
-    RPFVM_BC_FIXED_VALUE = 4
+    RPFVM_BC_FIXED_VALUE = 4,
+    /* Double-Mach-reflection sides (Woodward & Colella), the two non-trivial sides of
+     * BC_DoubleMach_2D (include/Boundary_Condition.h:480-660), which the reference's
+     * steppers leave commented out (include/time_advance.h:591-620,826-861).  Both are
+     * evaluated at the time of the RK stage being filled.  Cell-centre position along
+     * the side: x_i = ((i - 4 + 0.5) / nx) * lx. */
+    RPFVM_BC_DMR_WALL = 5,   /* bottom only: x_i <= bc_par[2][0]: fixed_state[2];
+                                beyond: mirrored reflection (v negated) */
+    RPFVM_BC_DMR_SHOCK = 6   /* top only: x_i < x_s(t): fixed_state[3], else
+                                bc_alt_state[3]; x_s(t) = p0 + (p1 / p2) + (p3 * t / p2)
+                                with p = bc_par[3] (start x, height, slope, speed) */
 };
 
 /* Riemann solver ids as the "scheme" control key, include/CFD_convection.h:89-117.
@@ -102,6 +112,9 @@ typedef struct rpfvm_config {
     double gamma, cfl, gravity;
     double inflow_state[4];    /* conserved (rho, rho u, rho v, E) */
     double fixed_state[4][4];  /* [side][var], side order left, right, bottom, top */
+    double lx;                 /* domain length in x for RPFVM_BC_DMR_* (0: nx * dx) */
+    double bc_par[4][4];       /* per-side parameters of RPFVM_BC_DMR_* */
+    double bc_alt_state[4][4]; /* per-side second state of RPFVM_BC_DMR_SHOCK */
 } rpfvm_config;
 
 typedef struct rpfvm_ctx rpfvm_ctx;
@@ -147,6 +160,12 @@ int rpfvm_space_operator(rpfvm_ctx *ctx, double *L, double *F, double *G);
  * t_stop or max_steps steps have run.  dt and t never leave the device inside the
  * loop; *t and *steps are updated on return.  Synchronous. */
 int rpfvm_advance_to(rpfvm_ctx *ctx, double t_stop, int max_steps, double *t, int *steps);
+
+/* Clock of the context, read by the time-dependent boundary types: rpfvm_time_advance
+ * fills stage k's ghosts at t + c_k dt (c = 0, 1, 1/2 for RK3; 0, 1 for RK2) and then
+ * advances the clock by dt; rpfvm_advance_to starts from *t.  Starts at 0. */
+int rpfvm_set_time(rpfvm_ctx *ctx, double t);
+int rpfvm_get_time(rpfvm_ctx *ctx, double *t);
 
 /* Wait for the context stream and return the fault bits accumulated since the last
  * call (0 when clean); the bits are cleared. */

@@ -35,8 +35,10 @@ def lib():
         L.orc_delta_t.restype = ctypes.c_double
         L.orc_delta_t.argtypes = [vp, dp]
         L.orc_boundary_conditions.argtypes = [vp, dp]
+        L.orc_boundary_conditions_t.argtypes = [vp, dp, ctypes.c_double]
         L.orc_space_operator.argtypes = [vp, dp, dp, dp, dp]
         L.orc_time_advance.argtypes = [vp, dp, ctypes.c_double]
+        L.orc_time_advance_t.argtypes = [vp, dp, ctypes.c_double, ctypes.c_double]
         L.orc_advance_to.argtypes = [vp, dp, ctypes.c_double, ctypes.c_int, dp, dp]
         L.orc_reconstruct.argtypes = [vp, dp, ctypes.c_int, dp, dp]
         L.orc_riemann.argtypes = [vp, ctypes.c_int, dp, dp, dp]
@@ -70,9 +72,9 @@ def delta_t(cfg, U):
     return lib().orc_delta_t(_cfgp(cfg), _ptr(U))
 
 
-def boundary_conditions(cfg, U):
-    """In place; returns U."""
-    _check(lib().orc_boundary_conditions(_cfgp(cfg), _ptr(U)))
+def boundary_conditions(cfg, U, t=0.0):
+    """In place at clock t (read by the time-dependent sides); returns U."""
+    _check(lib().orc_boundary_conditions_t(_cfgp(cfg), _ptr(U), ctypes.c_double(t)))
     return U
 
 
@@ -85,9 +87,10 @@ def space_operator(cfg, U):
     return L, F, G, rc
 
 
-def time_advance(cfg, U, dt):
-    """One RK step in place; returns fault bits."""
-    return _check(lib().orc_time_advance(_cfgp(cfg), _ptr(U), ctypes.c_double(dt)))
+def time_advance(cfg, U, dt, t=0.0):
+    """One RK step in place from clock t; returns fault bits."""
+    return _check(lib().orc_time_advance_t(_cfgp(cfg), _ptr(U), ctypes.c_double(dt),
+                                           ctypes.c_double(t)))
 
 
 def advance_to(cfg, U, t_stop, max_steps, t=0.0):

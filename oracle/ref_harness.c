@@ -21,6 +21,7 @@
  *             per step: dt = Get_Delta_T_2D (or fixed_dt), RK{1,2,3}_TimeAd
  *             (src/RP_FVM.c:182, 193-209)
  *   bc        in: U; out: U after Boundary_Conditions (include/Boundary_Condition.h:963)
+ *   dmrbc     in: U; out: U after BC_DoubleMach_2D at `time` (include/Boundary_Condition.h:480)
  *   residual  in: U; out: U (after BC), L[4][R][C], F[4][R][C], G[4][R][C]
  *             Source_Gravity, Boundary_Conditions, Flux_Reconstruction_RP,
  *             Space_Discrete_Item (include/time_advance.h:824-829)
@@ -58,7 +59,7 @@ typedef struct {
     int nx, ny, steps, time_order, scheme, recon, characteriz, source;
     int bc[4];
     int n; /* weno: number of stencils */
-    double gamma, cfl, dx, dy, gravity, fixed_dt;
+    double gamma, cfl, dx, dy, gravity, fixed_dt, time;
     double inflow[4];
     double fixed[4][4];
 } job_t;
@@ -101,6 +102,7 @@ static void read_job(const char *path, job_t *j) {
         else if (!strcmp(key, "dy")) j->dy = strtod(val, NULL);
         else if (!strcmp(key, "gravity")) j->gravity = strtod(val, NULL);
         else if (!strcmp(key, "fixed_dt")) j->fixed_dt = strtod(val, NULL);
+        else if (!strcmp(key, "time")) j->time = strtod(val, NULL);
         else if (!strncmp(key, "inflow", 6)) j->inflow[atoi(key + 6)] = strtod(val, NULL);
         else if (!strncmp(key, "fixed", 5)) {
             int side = key[5] - '0', comp = key[6] - '0';
@@ -223,6 +225,9 @@ int main(int argc, char **argv) {
         write_doubles(out, &dt, 1);
     } else if (!strcmp(j.mode, "bc")) {
         Boundary_Conditions(var, rows, cols, U, GhostCell);
+        write_doubles(out, ubuf, nU);
+    } else if (!strcmp(j.mode, "dmrbc")) {
+        BC_DoubleMach_2D(var, rows, cols, U, GhostCell, j.time);
         write_doubles(out, ubuf, nU);
     } else if (!strcmp(j.mode, "residual")) {
         grid_t *F = calloc(nU, sizeof(double)), *G = calloc(nU, sizeof(double));

@@ -612,18 +612,60 @@ static int check_cfg_(const rpfvm_config *c) {
     if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
         return RPFVM_ERR_UNSUPPORTED;
     for (int s = 0; s < 4; s++)
-        if (c->bc[s] < 0 || c->bc[s] > 4) return RPFVM_ERR_INVALID;
+        if (c->bc[s] < 0 || c->bc[s] > 6) return RPFVM_ERR_INVALID;
+    /* the double-Mach-reflection sides exist on the bottom / top only */
+    for (int s = 0; s < 4; s++)
+        if ((c->bc[s] == RPFVM_BC_DMR_WALL && s != 2) || (c->bc[s] == RPFVM_BC_DMR_SHOCK && s != 3))
+            return RPFVM_ERR_INVALID;
     return 0;
+}
+
+/* Cell-centre x of row i for the double-Mach-reflection sides, in the form of
+ * BC_DoubleMach_2D (include/Boundary_Condition.h:564): ((i + 0.5 - GC) / nx) * Lx. */
+static double dmr_x_(const rpfvm_config *cfg, int i) {
+    const double lx = cfg->lx > 0.0 ? cfg->lx : cfg->nx * cfg->dx;
+    return (double)(i + 0.5 - GC) / (double)cfg->nx * lx;
+}
+
+/* The bottom and top sides of BC_DoubleMach_2D (include/Boundary_Condition.h:558-660)
+ * at time t: the bottom is the fixed state up to the wall start and a mirrored
+ * reflection beyond it; the top is the fixed state left of the moving shock trace and
+ * the second state right of it.  Rows i = GC .. R-GC-1 only, as the reference. */
+static void bc_dmr_(const rpfvm_config *cfg, double *x, int side, double t) {
+    const dims_t d = dims_of(cfg);
+    const double *p = cfg->bc_par[side];
+    for (int i = GC; i < d.R - GC; i++) {
+        const double xp = dmr_x_(cfg, i);
+        for (int v = 0; v < 4; v++) {
+            for (int m = 0; m < 4; m++) {
+                if (side == 2) {
+                    if (xp > p[0]) {
+                        const double s = AT(x, v, i, GC + m);
+                        AT(x, v, i, GC - 1 - m) = v == 2 ? -s : s;
+                    } else {
+                        AT(x, v, i, GC - 1 - m) = cfg->fixed_state[2][v];
+                    }
+                } else {
+                    const double xs = p[0] + (p[1] / p[2]) + (p[3] * t / p[2]);
+                    AT(x, v, i, d.C - GC + m) = xp < xs ? cfg->fixed_state[3][v] : cfg->bc_alt_state[3][v];
+                }
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------ ghost-cell fill */
 /* One side of Boundary_Conditions; index maps of include/Boundary_Condition.h:
  * fixed 63-135, inflow 148-202, outflow 216-270, periodic 283-336, reflection
  * 350-465 (left/top mirrored, right/bottom copy the first interior cell). */
-static void bc_side_(const rpfvm_config *cfg, double *x, int side) {
+static void bc_side_(const rpfvm_config *cfg, double *x, int side, double time) {
     const dims_t d = dims_of(cfg);
     const int R = d.R, C = d.C;
     const int type = cfg->bc[side];
+    if (type == RPFVM_BC_DMR_WALL || type == RPFVM_BC_DMR_SHOCK) {
+        bc_dmr_(cfg, x, side, time);
+        return;
+    }
     const int lr = side < 2;                       /* left/right walk k over columns */
     const int lo = GC;
     const int hi = type == RPFVM_BC_OUTFLOW ? (lr ? C - GC : R - GC)   /* inclusive */
@@ -672,11 +714,15 @@ static void bc_side_(const rpfvm_config *cfg, double *x, int side) {
     }
 }
 
-int orc_boundary_conditions(const rpfvm_config *cfg, double *U) {
+int orc_boundary_conditions_t(const rpfvm_config *cfg, double *U, double t) {
     int e = check_cfg_(cfg);
     if (e) return e;
-    for (int side = 0; side < 4; side++) bc_side_(cfg, U, side);
+    for (int side = 0; side < 4; side++) bc_side_(cfg, U, side, t);
     return 0;
+}
+
+int orc_boundary_conditions(const rpfvm_config *cfg, double *U) {
+    return orc_boundary_conditions_t(cfg, U, 0.0);
 }
 
 /* ---------------------------------------------------------------- dt (CFL) */
@@ -783,9 +829,9 @@ static void work_free_(work_t *w) {
 }
 
 /* One stage operator on Y: source, ghost fill in place, fluxes, residual into w->L. */
-static int stage_operator_(const rpfvm_config *cfg, double *Y, work_t *w) {
+static int stage_operator_(const rpfvm_config *cfg, double *Y, work_t *w, double t) {
     if (cfg->source) source_(cfg, Y, w->S);
-    for (int side = 0; side < 4; side++) bc_side_(cfg, Y, side);
+    for (int side = 0; side < 4; side++) bc_side_(cfg, Y, side, t);
     int bad = fluxes_(cfg, Y, w->F, w->G);
     residual_(cfg, w->F, w->G, w->S, w->L);
     return bad;
@@ -796,7 +842,7 @@ int orc_space_operator(const rpfvm_config *cfg, double *U, double *L, double *F,
     if (e) return e;
     work_t w;
     if (work_alloc_(cfg, &w)) return RPFVM_ERR_NOMEM;
-    int bad = stage_operator_(cfg, U, &w);
+    int bad = stage_operator_(cfg, U, &w, 0.0);
     if (L) memcpy(L, w.L, w.n * sizeof(double));
     if (F) memcpy(F, w.F, w.n * sizeof(double));
     if (G) memcpy(G, w.G, w.n * sizeof(double));
@@ -917,9 +963,9 @@ static int repair_(const rpfvm_config *cfg, const double *Un, const double *Y, d
 
 /* One RK stage: out = combine(kind, Un, Y, dt L(Y)).  out may alias Un. */
 static int stage_(const rpfvm_config *cfg, const double *Un, double *Y, double *out, int kind,
-                  double dt, work_t *w, double *tmp, unsigned char *state) {
+                  double dt, double t, work_t *w, double *tmp, unsigned char *state) {
     const dims_t d = dims_of(cfg);
-    int bad = stage_operator_(cfg, Y, w);
+    int bad = stage_operator_(cfg, Y, w, t);
     double *dst = cfg->positivity ? tmp : out;
 #pragma omp parallel for collapse(2)
     for (int i = GC; i <= d.R - GC - 1; i++)
@@ -937,6 +983,11 @@ static int stage_(const rpfvm_config *cfg, const double *Un, double *Y, double *
 }
 
 int orc_time_advance(const rpfvm_config *cfg, double *U, double dt) {
+    return orc_time_advance_t(cfg, U, dt, 0.0);
+}
+
+/* Stage k fills its ghosts at t + c_k dt: SSP-RK3 abscissae c = (0, 1, 1/2), RK2 (0, 1). */
+int orc_time_advance_t(const rpfvm_config *cfg, double *U, double dt, double t) {
     int e = check_cfg_(cfg);
     if (e) return e;
     work_t w;
@@ -952,14 +1003,14 @@ int orc_time_advance(const rpfvm_config *cfg, double *U, double dt) {
     g_repairs = 0;
     /* RK1_TimeAd :643-708, RK2_TimeAd :711-785, RK3_TimeAd :788-882 */
     if (cfg->time_order == 1) {
-        bad |= stage_(cfg, U, U, U, K_EULER, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U, U, K_EULER, dt, t, &w, tmp, state);
     } else if (cfg->time_order == 2) {
-        bad |= stage_(cfg, U, U, U1, K_EULER, dt, &w, tmp, state);
-        bad |= stage_(cfg, U, U1, U, K_RK2_2, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U, U1, K_EULER, dt, t, &w, tmp, state);
+        bad |= stage_(cfg, U, U1, U, K_RK2_2, dt, t + dt, &w, tmp, state);
     } else {
-        bad |= stage_(cfg, U, U, U1, K_EULER, dt, &w, tmp, state);
-        bad |= stage_(cfg, U, U1, U2, K_RK3_2, dt, &w, tmp, state);
-        bad |= stage_(cfg, U, U2, U, K_RK3_3, dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U, U1, K_EULER, dt, t, &w, tmp, state);
+        bad |= stage_(cfg, U, U1, U2, K_RK3_2, dt, t + dt, &w, tmp, state);
+        bad |= stage_(cfg, U, U2, U, K_RK3_3, dt, t + 0.5 * dt, &w, tmp, state);
     }
     free(U1);
     free(U2);
@@ -981,7 +1032,7 @@ int orc_advance_to(const rpfvm_config *cfg, double *U, double t_stop, int max_st
         if (time + dt > t_stop) dt = t_stop - time;
         if (time + dt >= t_stop) dt = t_stop - time;
         if (dts) dts[n] = dt;
-        orc_time_advance(cfg, U, dt);
+        orc_time_advance_t(cfg, U, dt, time);
         time = time + dt;
         n++;
     }

@@ -22,6 +22,8 @@ double orc_delta_t(const rpfvm_config *cfg, const double *U);
 
 /* Boundary_Conditions (include/Boundary_Condition.h:963-980), in place. */
 int orc_boundary_conditions(const rpfvm_config *cfg, double *U);
+/* The same at clock t (time-dependent sides). */
+int orc_boundary_conditions_t(const rpfvm_config *cfg, double *U, double t);
 
 /* Source_Gravity + Boundary_Conditions (in place on U) + Flux_Reconstruction_RP +
  * Space_Discrete_Item.  L, F, G are full [4][R][C] arrays (any may be NULL). */
@@ -29,6 +31,8 @@ int orc_space_operator(const rpfvm_config *cfg, double *U, double *L, double *F,
 
 /* RK1/2/3_TimeAd (include/time_advance.h:643-882), in place on U. */
 int orc_time_advance(const rpfvm_config *cfg, double *U, double dt);
+/* The same starting at clock t: stage k fills its ghosts at t + c_k dt. */
+int orc_time_advance_t(const rpfvm_config *cfg, double *U, double dt, double t);
 
 /* Driver time loop (src/RP_FVM.c:179-191) with one output time t_stop: steps until
  * t reaches t_stop or max_steps; returns steps taken, *t updated, dts[] optional. */

@@ -17,10 +17,11 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-LIBDIR = os.path.join(PKG, "lib")
+LIBDIR = os.environ.get("RPFVM_LIB_DIR", os.path.join(PKG, "lib"))
 LIB = os.path.join(LIBDIR, "librpfvm.so")
 PROBE = os.path.join(LIBDIR, "hbm_probe")
-BUILD = os.path.join(PKG, "_build")
+DRIVER = os.path.join(LIBDIR, "rp_fvm_gpu")
+BUILD = os.environ.get("RPFVM_BUILD_DIR", os.path.join(PKG, "_build"))
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -30,13 +31,16 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract
           "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable"]
 
+# extra compiler flags, e.g. RPFVM_HIPFLAGS="-DRPFVM_FLUX_WAVES=4"
+EXTRA = os.environ.get("RPFVM_HIPFLAGS", "").split()
+
 SOURCES = ["fvm_kernels.hip", "fvm_mixed_kernels.hip", "fvm_batch_kernels.hip", "fvm_api.cpp",
            "fvm_batch_api.cpp"]
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *EXTRA, "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd.insert(1, "-xhip")
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,8 +65,15 @@ def build(verbose: bool = True) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # the C driver (host code only, plain gcc; finds librpfvm.so next to itself)
+    cmd = ["gcc", "-O2", "-std=gnu11", "-ffp-contract=off", "-Wall", "-I",
+           os.path.join(ROOT, "include"), os.path.join(PKG, "driver", "rp_fvm_gpu.c"),
+           "-L", LIBDIR, "-lrpfvm", "-Wl,-rpath,$ORIGIN", "-lm", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB} and {PROBE}")
+        print(f"built {LIB}, {PROBE} and {DRIVER}")
     return LIB
 
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "rpfvm_get_state", "rpfvm_get_delta_t", "rpfvm_time_advance", "rpfvm_boundary_conditions",
     "rpfvm_space_operator", "rpfvm_advance_to", "rpfvm_sync", "rpfvm_get_faults",
     "rpfvm_device_state", "rpfvm_stream", "rpfvm_set_kernel_timing", "rpfvm_kernel_times",
-    "rpfvm_last_error", "rpfvm_version",
+    "rpfvm_last_error", "rpfvm_version", "rpfvm_set_time", "rpfvm_get_time",
     "rpfvm_batch_create", "rpfvm_batch_destroy", "rpfvm_batch_set_state", "rpfvm_batch_get_state",
     "rpfvm_batch_step", "rpfvm_batch_advance_to", "rpfvm_batch_status", "rpfvm_batch_sync",
     "rpfvm_batch_tune", "rpfvm_batch_set_tuning", "rpfvm_batch_get_tuning", "rpfvm_batch_stream",
@@ -87,6 +87,8 @@ def lib():
     L.rpfvm_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.rpfvm_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.rpfvm_kernel_times.argtypes = [vp, dp, ip]
+    L.rpfvm_set_time.argtypes = [vp, ctypes.c_double]
+    L.rpfvm_get_time.argtypes = [vp, dp]
     tp = ctypes.POINTER(BatchTuning)
     L.rpfvm_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(Config), ctypes.POINTER(vp)]
     L.rpfvm_batch_destroy.argtypes = [vp]
@@ -182,6 +184,15 @@ class Solver:
         G = np.zeros(self.shape)
         _check(lib().rpfvm_space_operator(self._ctx, _dptr(L), _dptr(F), _dptr(G)))
         return L, F, G
+
+    def set_time(self, t: float) -> None:
+        """Clock read by the time-dependent boundary sides (rpfvm_set_time)."""
+        _check(lib().rpfvm_set_time(self._ctx, ctypes.c_double(t)))
+
+    def get_time(self) -> float:
+        t = ctypes.c_double()
+        _check(lib().rpfvm_get_time(self._ctx, ctypes.byref(t)))
+        return t.value
 
     def advance_to(self, t_stop: float, max_steps: int, t: float = 0.0):
         tt = ctypes.c_double(t)

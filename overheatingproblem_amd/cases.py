@@ -18,8 +18,8 @@ import math
 
 import numpy as np
 
-from .config import (BC_OUTFLOW, BC_REFLECTION, BC_PERIODICITY, BC_INFLOW, BC_FIXED_VALUE,
-                     GHOST, SCHEME_HLLC, make_config)
+from .config import (BC_DMR_SHOCK, BC_DMR_WALL, BC_FIXED_VALUE, BC_INFLOW, BC_OUTFLOW,
+                     BC_PERIODICITY, BC_REFLECTION, GHOST, SCHEME_HLLC, make_config)
 
 
 def prim_to_cons(rho, u, v, p, gamma):
@@ -128,10 +128,9 @@ def sedov(n=200, *, gamma=1.4, e0=1.0, p_bg=1.0e-5, r_dep=2.0, **cfg_kw):
 
 
 def double_mach(nx=8192, ny=2048, *, gamma=1.4, **cfg_kw):
-    """Mach-10 double Mach reflection on [0,4]x[0,1] (Woodward & Colella 1984):
-    shock at 60 degrees through (1/6, 0).  Boundary states follow the
-    BC_DoubleMach_2D specification (include/Boundary_Condition.h:480-692) with the
-    exact post-shock velocity 8.25 (cos 30, -sin 30)."""
+    """Mach-10 double Mach reflection on [0,4]x[0,1] (Woodward & Colella 1984) with
+    static boundaries (fixed post-shock left and top, reflecting bottom): the
+    short-run parity workload.  :func:`dmr` has the moving boundaries."""
     dx, dy = 4.0 / nx, 1.0 / ny
     X, Y = cell_centres(nx, ny, dx, dy)
     post = (8.0, 8.25 * math.cos(math.pi / 6), -8.25 * math.sin(math.pi / 6), 116.5)
@@ -146,6 +145,40 @@ def double_mach(nx=8192, ny=2048, *, gamma=1.4, **cfg_kw):
     cfg = make_config(nx, ny, dx=dx, dy=dy, gamma=gamma,
                       bc=(BC_FIXED_VALUE, BC_OUTFLOW, BC_REFLECTION, BC_FIXED_VALUE),
                       fixed_state=(cons_post, cons_post, cons_post, cons_post), **cfg_kw)
+    return cfg, U
+
+
+def dmr(nx=8192, ny=2048, *, gamma=1.4, positivity=True, **cfg_kw):
+    """Mach-10 double Mach reflection (Woodward & Colella 1984) on [0,4]x[0,1] with the
+    moving boundaries of BC_DoubleMach_2D (include/Boundary_Condition.h:480-660): post-shock
+    inflow on the left, outflow on the right, the bottom post-shock up to x = 1/6 and a
+    reflecting wall beyond, the top post-shock left of the shock trace
+    x_s(t) = 1/6 + (1 + 20 t)/sqrt(3) and pre-shock right of it.  Pre-shock
+    (1.4, 0, 0, 1), post-shock (8, 8.25 cos 30, -8.25 sin 30, 116.5); initial shock
+    y = sqrt(3)(x - 1/6) as initEulerpri2D_DoubleMachReflection
+    (include/initialize.h:890-966).  WENO5-char + Roe, RK3, CFL 0.5, with the positivity
+    safeguard (first-order face fallback + HLLE cell repair) as the HLL fallback."""
+    dx, dy = 4.0 / nx, 1.0 / ny
+    X, Y = cell_centres(nx, ny, dx, dy)
+    post = (8.0, 8.25 * math.cos(math.pi / 6), -8.25 * math.sin(math.pi / 6), 116.5)
+    pre = (1.4, 0.0, 0.0, 1.0)
+    ahead = Y < math.sqrt(3.0) * (X - 1.0 / 6.0)
+    prim = [np.where(ahead, pre[k], post[k]) for k in range(4)]
+    U = prim_to_cons(*prim, gamma)
+    cpost = tuple(float(x) for x in prim_to_cons(*post, gamma))
+    cpre = tuple(float(x) for x in prim_to_cons(*pre, gamma))
+    cfg_kw.setdefault("scheme", 3)
+    cfg_kw.setdefault("characteristic", True)
+    cfg_kw.setdefault("cfl", 0.5)
+    cfg = make_config(nx, ny, dx=dx, dy=dy, gamma=gamma, positivity=positivity,
+                      bc=(BC_INFLOW, BC_OUTFLOW, BC_DMR_WALL, BC_DMR_SHOCK), inflow_state=cpost,
+                      fixed_state=(cpost, cpost, cpost, cpost), **cfg_kw)
+    cfg.lx = 4.0
+    cfg.bc_par[2][0] = 1.0 / 6.0
+    for k, v in enumerate((1.0 / 6.0, 1.0, math.sqrt(3.0), 20.0)):
+        cfg.bc_par[3][k] = v
+    for k in range(4):
+        cfg.bc_alt_state[3][k] = cpre[k]
     return cfg, U
 
 

@@ -11,6 +11,7 @@ import ctypes
 GHOST = 4
 
 BC_OUTFLOW, BC_REFLECTION, BC_PERIODICITY, BC_INFLOW, BC_FIXED_VALUE = range(5)
+BC_DMR_WALL, BC_DMR_SHOCK = 5, 6
 BC_NAMES = {"outflow": BC_OUTFLOW, "reflection": BC_REFLECTION, "periodicity": BC_PERIODICITY,
             "inflow": BC_INFLOW, "fixed": BC_FIXED_VALUE}
 
@@ -51,6 +52,9 @@ class Config(ctypes.Structure):
         ("gravity", ctypes.c_double),
         ("inflow_state", ctypes.c_double * 4),
         ("fixed_state", (ctypes.c_double * 4) * 4),
+        ("lx", ctypes.c_double),
+        ("bc_par", (ctypes.c_double * 4) * 4),
+        ("bc_alt_state", (ctypes.c_double * 4) * 4),
     ]
 
     @property
@@ -79,6 +83,8 @@ class Config(ctypes.Structure):
             "dx": self.dx, "dy": self.dy, "gamma": self.gamma, "cfl": self.cfl,
             "gravity": self.gravity, "inflow_state": list(self.inflow_state),
             "fixed_state": [list(r) for r in self.fixed_state],
+            "lx": self.lx, "bc_par": [list(r) for r in self.bc_par],
+            "bc_alt_state": [list(r) for r in self.bc_alt_state],
         }
 
 
@@ -116,4 +122,9 @@ def config_from_dict(d: dict) -> Config:
                     bc=d["bc"], gamma=d["gamma"], cfl=d["cfl"],
                     gravity=d["gravity"], inflow_state=d["inflow_state"],
                     fixed_state=d["fixed_state"])
+    c.lx = float(d.get("lx", 0.0))
+    for s in range(4):
+        for k in range(4):
+            c.bc_par[s][k] = float(d.get("bc_par", [[0.0] * 4] * 4)[s][k])
+            c.bc_alt_state[s][k] = float(d.get("bc_alt_state", [[0.0] * 4] * 4)[s][k])
     return c

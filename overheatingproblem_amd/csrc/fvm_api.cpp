@@ -62,6 +62,7 @@ struct rpfvm_ctx {
     double* host_scal = nullptr;   // pinned mirror of scal
     unsigned* host_faults = nullptr;
     unsigned host_pending_faults = 0;  // fault bits found on the host (advance_to)
+    double time = 0.0;                 // clock of the time-dependent boundary sides
     // kernel timing (rpfvm_set_kernel_timing)
     bool timing = false;
     struct Pending {
@@ -121,7 +122,10 @@ int fvm::api::validate(const rpfvm_config* c) {
     if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
         return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
     for (int s = 0; s < 4; s++)
-        if (c->bc[s] < 0 || c->bc[s] > 4) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
+        if (c->bc[s] < 0 || c->bc[s] > 6) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
+    if (c->bc[0] >= RPFVM_BC_DMR_WALL || c->bc[1] >= RPFVM_BC_DMR_WALL ||
+        c->bc[2] == RPFVM_BC_DMR_SHOCK || c->bc[3] == RPFVM_BC_DMR_WALL)
+        return fail(RPFVM_ERR_INVALID, "the double-Mach-reflection wall is a bottom side, the shock a top side");
     if (!(c->dx > 0) || !(c->dy > 0) || !(c->gamma > 1.0))
         return fail(RPFVM_ERR_INVALID, "dx, dy must be > 0 and gamma > 1");
     return 0;
@@ -188,8 +192,14 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
         const bool lr = s < 2;
         const int last = lr ? x->g.C - GC : x->g.R - GC;  // outflow loops are inclusive
         b.hi = b.type == BC_OUT ? last : last - 1;
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 4; k++) {
             b.value[k] = b.type == BC_INFLOW ? c.inflow_state[k] : c.fixed_state[s][k];
+            b.alt[k] = c.bc_alt_state[s][k];
+            b.par[k] = c.bc_par[s][k];
+        }
+        b.nx = (double)c.nx;
+        b.lx = c.lx > 0.0 ? c.lx : c.nx * c.dx;
+        b.t = 0.0;
     }
 
     auto cleanup = [&](int code) {
@@ -293,9 +303,13 @@ int rpfvm_get_state(rpfvm_ctx* x, double* U) {
     return copy_out(x, U, x->U);
 }
 
-static void enqueue_bc(rpfvm_ctx* x, double* Y) {
-    Timed t(x, RPFVM_KCLASS_BC);
-    for (int s = 0; s < 4; s++) launch_bc_side(Y, x->g, x->bc[s], x->stream);
+// Ghost fill of Y at clock t (read only by the double-Mach-reflection sides).
+static void enqueue_bc(rpfvm_ctx* x, double* Y, double t) {
+    Timed tm(x, RPFVM_KCLASS_BC);
+    for (int s = 0; s < 4; s++) {
+        x->bc[s].t = t;
+        launch_bc_side(Y, x->g, x->bc[s], x->stream);
+    }
 }
 
 static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
@@ -324,9 +338,9 @@ static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
 // and stage combine into out (separately when out aliases Y, which the fused kernel's
 // neighbour reads forbid), then the positivity safeguard.
 static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
-                          int kind) {
+                          int kind, double t_stage) {
     const unsigned stamp = ++x->stamp;
-    enqueue_bc(x, Y);
+    enqueue_bc(x, Y, t_stage);
     {
         Timed t(x, RPFVM_KCLASS_FLUX_X);
         x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
@@ -355,20 +369,23 @@ static int enqueue_step(rpfvm_ctx* x, double dt) {
     // which takes over U's ghost frame (the reference leaves U's ghosts as filled at
     // stage 1) and becomes the state.
     double* last = x->U3 ? x->U3 : x->U;
+    // stage k fills its ghosts at t + c_k dt (SSP-RK3: c = 0, 1, 1/2; RK2: 0, 1)
+    const double t = x->time;
     switch (x->cfg.time_order) {
         case 1:
-            enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER);
+            enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER, t);
             break;
         case 2:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
-            enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2);
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t);
+            enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2, t + dt);
             break;
         default:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER);
-            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2);
-            enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3);
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t);
+            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2, t + dt);
+            enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3, t + 0.5 * dt);
             break;
     }
+    x->time = t + dt;
     if (last != x->U) {
         launch_copy_frame(x->U, last, x->g, x->stream);
         x->U3 = x->U;
@@ -385,8 +402,20 @@ int rpfvm_time_advance(rpfvm_ctx* x, double dt) {
 
 int rpfvm_boundary_conditions(rpfvm_ctx* x) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
-    enqueue_bc(x, x->U);
+    enqueue_bc(x, x->U, x->time);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int rpfvm_set_time(rpfvm_ctx* x, double t) {
+    if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    x->time = t;
+    return 0;
+}
+
+int rpfvm_get_time(rpfvm_ctx* x, double* t) {
+    if (!x || !t) return fail(RPFVM_ERR_INVALID, "null argument");
+    *t = x->time;
     return 0;
 }
 
@@ -402,7 +431,7 @@ int rpfvm_get_delta_t(rpfvm_ctx* x, double* dt) {
 
 int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
-    enqueue_bc(x, x->U);
+    enqueue_bc(x, x->U, x->time);
     enqueue_fluxes(x, x->U);
     enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, Repair{});
     HIPCHK(hipGetLastError());
@@ -417,6 +446,7 @@ int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
 int rpfvm_advance_to(rpfvm_ctx* x, double t_stop, int max_steps, double* t, int* steps) {
     if (!x || !t) return fail(RPFVM_ERR_INVALID, "null argument");
     double time = *t;
+    x->time = time;
     int n = 0;
     // src/RP_FVM.c:179-191 with next_output_time == Tmax == t_stop
     while (time < t_stop && n < max_steps) {
@@ -433,7 +463,7 @@ int rpfvm_advance_to(rpfvm_ctx* x, double t_stop, int max_steps, double* t, int*
             break;
         }
         if ((e = enqueue_step(x, dt))) return e;
-        time = time + dt;
+        time = time + dt;  // == x->time
         n++;
     }
     HIPCHK(hipStreamSynchronize(x->stream));

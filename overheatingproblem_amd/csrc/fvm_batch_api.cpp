@@ -148,6 +148,9 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: the batch form runs recon_order 5", k);
         if (c.positivity)
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: no positivity safeguard in a batch", k);
+        for (int s = 0; s < 4; s++)
+            if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
+                return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);
         // The mixed-precision batch stage (k_batch_stage<..., MIXED> + k_batch_fix) did
         // not complete its first step on the GPU and the cause is not yet identified;
         // it stays switched off until it is.

@@ -28,6 +28,21 @@ __global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s) {
     const int v = blockIdx.y;
     if (t > s.hi) return;
     double* P = U + (size_t)v * g.plane;
+    if (s.type == BC_DMR_WALL || s.type == BC_DMR_SHOCK) {
+        // bottom / top of BC_DoubleMach_2D (include/Boundary_Condition.h:558-660), row t
+        const double xp = (double)(t + 0.5 - GC) / s.nx * s.lx;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (s.type == BC_DMR_WALL) {
+                const double x = P[(size_t)t * g.ld + GC + m];
+                P[(size_t)t * g.ld + GC - 1 - m] = xp > s.par[0] ? (v == 2 ? -x : x) : s.value[v];
+            } else {
+                const double xs = s.par[0] + (s.par[1] / s.par[2]) + (s.par[3] * s.t / s.par[2]);
+                P[(size_t)t * g.ld + g.C - GC + m] = xp < xs ? s.value[v] : s.alt[v];
+            }
+        }
+        return;
+    }
     const bool lr = s.side < 2;
     const double sign = (s.type == BC_REFLECT && v == (lr ? 1 : 2)) ? -1.0 : 1.0;
 #pragma unroll
@@ -69,7 +84,7 @@ __global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s) {
 // DIR 2: faces (i, j+1/2), i = GC..R-GC-1, j = GC-1..C-GC-1.  The flux is stored at
 // the face's left cell, [i][j], as the reference's Flux array.
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(256) void k_flux(const double* __restrict__ Y, double* __restrict__ F,
+__global__ FLUX_BOUNDS void k_flux(const double* __restrict__ Y, double* __restrict__ F,
                                               Geom g, Phys ph, unsigned* __restrict__ faults) {
     const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
     const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
@@ -120,7 +135,7 @@ struct FaceOp64 {
 };
 
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(256) void k_yflux_update(const double* Un, const double* Y,
+__global__ FLUX_BOUNDS void k_yflux_update(const double* Un, const double* Y,
                                                       double* out, const double* __restrict__ F,
                                                       double* G, Geom g, Phys ph, double dt,
                                                       int kind, Repair rp,

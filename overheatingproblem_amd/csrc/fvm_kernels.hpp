@@ -9,6 +9,17 @@ namespace fvm {
 
 constexpr int GC = 4;  // GhostCell, include/Golbal.h:49
 
+// Minimum resident wavefronts per SIMD the face-flux kernels are compiled for (0: the
+// compiler's choice).  4 caps them at 128 VGPRs (with spills); build-time tuning knob.
+#ifndef RPFVM_FLUX_WAVES
+#define RPFVM_FLUX_WAVES 0
+#endif
+#if RPFVM_FLUX_WAVES > 0
+#define FLUX_BOUNDS __launch_bounds__(256, RPFVM_FLUX_WAVES)
+#else
+#define FLUX_BOUNDS __launch_bounds__(256)
+#endif
+
 // Device grid geometry: planes of R rows x ld doubles; plane v at base + v*plane.
 struct Geom {
     int R, C;       // rows (x, with ghosts) and logical columns (y, with ghosts)
@@ -24,13 +35,20 @@ struct Phys {
     int force_fix;  // mixed form: send every face to the FP64 recompute list
 };
 
-enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4 };
+enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4,
+             BC_DMR_WALL = 5, BC_DMR_SHOCK = 6 };
 
 // One side of the ghost fill: side 0..3 = left, right, bottom, top; tangential index
-// range [lo, hi] (inclusive); value[] is the inflow or fixed state when used.
+// range [lo, hi] (inclusive); value[] is the inflow or fixed state when used.  The
+// double-Mach-reflection sides also use alt[] (state right of the shock), par[] (wall
+// start, or shock start / height / slope / speed), the cell-centre map x_i =
+// ((i + 0.5 - GC) / nx) * lx and the stage time t.
 struct BcSide {
     int side, type, lo, hi;
     double value[4];
+    double alt[4];
+    double par[4];
+    double nx, lx, t;
 };
 
 enum : int { UPD_RESIDUAL = 0, UPD_EULER = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };

@@ -31,7 +31,7 @@ __device__ __forceinline__ int face_fp64(const Phys& ph, const double st[6][4], 
 }
 
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(256) void k_flux_mixed(const double* __restrict__ Y,
+__global__ FLUX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
                                                     double* __restrict__ F, Geom g, Phys ph,
                                                     FaceFix fix) {
     const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
@@ -103,7 +103,7 @@ struct FaceOpMx {
 };
 
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(256) void k_yflux_update_mixed(const double* Un, const double* Y,
+__global__ FLUX_BOUNDS void k_yflux_update_mixed(const double* Un, const double* Y,
                                                             double* out,
                                                             const double* __restrict__ F,
                                                             double* G, Geom g, Phys ph, double dt,

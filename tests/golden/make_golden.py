@@ -259,12 +259,52 @@ def make_dt():
     return out
 
 
+def dmr_reference_config(nx, ny):
+    """The constants BC_DoubleMach_2D hard-codes (include/Boundary_Condition.h:483-512):
+    [0,4]x[0,1], gamma 1.4, pre-shock (1.4, 0, 0, 1), post-shock (8, 7.1447, -4.125,
+    116.5), wall and shock start at x = 1/6, slope sqrt(3), shock speed 20 -- as the
+    double-Mach-reflection sides of this repository's configuration."""
+    import math
+    post = tuple(float(x) for x in cases.prim_to_cons(8.0, 7.1447, -4.125, 116.5, 1.4))
+    pre = tuple(float(x) for x in cases.prim_to_cons(1.4, 0.0, 0.0, 1.0, 1.4))
+    cfg = make_config(nx, ny, dx=4.0 / nx, dy=1.0 / ny, bc=(3, 0, 5, 6), inflow_state=post,
+                      fixed_state=(post, post, post, post))
+    cfg.lx = 4.0
+    cfg.bc_par[2][0] = 1.0 / 6.0
+    for k, v in enumerate((1.0 / 6.0, 1.0, math.sqrt(3.0), 20.0)):
+        cfg.bc_par[3][k] = v
+    for k in range(4):
+        cfg.bc_alt_state[3][k] = pre[k]
+    return cfg
+
+
+def make_dmrbc():
+    """BC_DoubleMach_2D on random states at several times (the shock trace sweeps the
+    top boundary; sizes put cell centres on both sides of the wall start)."""
+    out = {}
+    rng = np.random.default_rng(7007)
+    for name, (nx, ny, t) in {"t0": (48, 12, 0.0), "t1": (48, 12, 0.013), "t2": (60, 9, 0.2),
+                              "t3": (24, 20, 0.11)}.items():
+        cfg = dmr_reference_config(nx, ny)
+        U = rng.uniform(0.5, 2.0, size=cfg.shape)
+        res = run(job_text("dmrbc", cfg, time=t), [U], U.size).reshape(cfg.shape)
+        out[name + "_cfg"] = cfg_json(cfg)
+        out[name + "_t"] = np.array(t)
+        out[name + "_U"] = U
+        out[name + "_out"] = res
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
-    for name, fn in (("weno", make_weno), ("flux", make_flux), ("eigen", make_eigen),
-                     ("recon", make_recon), ("bc", make_bc), ("residual", make_residual),
-                     ("steps", make_steps), ("dt", make_dt)):
+    makers = (("weno", make_weno), ("flux", make_flux), ("eigen", make_eigen),
+              ("recon", make_recon), ("bc", make_bc), ("residual", make_residual),
+              ("steps", make_steps), ("dt", make_dt), ("dmrbc", make_dmrbc))
+    only = set(sys.argv[1:])  # fixture names to (re)write; all when empty
+    for name, fn in makers:
+        if only and name not in only:
+            continue
         data = fn()
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **data)

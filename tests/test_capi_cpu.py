@@ -45,7 +45,7 @@ def test_config_struct_layout_matches_header(lib):
     assert c.characteristic == 0 and c.positivity == 0 and list(c.bc) == [0, 0, 0, 0]
     assert c.gamma == 1.4 and c.cfl == 0.5 and c.dx == 0.01
     assert c.inflow_state[1] == 1.4 * 3.0 and c.fixed_state[3][3] == 1.0 / 0.4 + 0.5 * 1.4 * 3.0 * 3.0
-    assert ctypes.sizeof(Config) == 14 * 4 + 5 * 8 + 4 * 8 + 16 * 8
+    assert ctypes.sizeof(Config) == 14 * 4 + 5 * 8 + 4 * 8 + 16 * 8 + 8 + 16 * 8 + 16 * 8
 
 
 def test_version(lib):
@@ -57,3 +57,18 @@ def test_make_config_roundtrip():
     c = make_config(10, 12, scheme=3, characteristic=True, bc=(1, 0, 2, 3), positivity=True)
     d = config_from_dict(c.as_dict())
     assert c.as_dict() == d.as_dict()
+
+
+def test_bench_noh_metrics_on_oracle_states(oracle):
+    """bench.py's accuracy report runs on small states (the oracle stands in for the
+    device forms here; both entries identical -> zero mixed-vs-FP64 error)."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    from overheatingproblem_amd import cases
+    cfg, U = cases.noh(32, positivity=True)
+    for _ in range(3):
+        oracle.time_advance(cfg, U, oracle.delta_t(cfg, U))
+    out = bench.noh_metrics({"fp64": U, "mixed": U.copy()}, 32, cfg.gamma)
+    assert out["mixed_vs_fp64"]["l1_rho"] == 0.0
+    assert 0.0 < out["fp64"]["rho_plateau_0.1_0.15"] < 20.0

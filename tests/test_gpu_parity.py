@@ -173,6 +173,43 @@ def test_advance_to_matches_driver_loop(oracle):
     assert same(G, U), first_diff(G, U)
 
 
+@pytest.mark.parametrize("key", ["t0", "t1", "t2", "t3"])
+def test_golden_double_mach_sides(key):
+    """Device double-Mach-reflection sides vs the reference's BC_DoubleMach_2D."""
+    g = load("dmrbc")
+    c = golden_cfg(g, key + "_cfg")
+    with Solver(c) as s:
+        s.set_state(g[key + "_U"])
+        s.set_time(float(g[key + "_t"]))
+        s.boundary_conditions()
+        G = s.get_state()
+    ref = g[key + "_out"]
+    R, C = G.shape[1:]
+    mask = np.ones((R, C), dtype=bool)
+    for a in (slice(0, 4), slice(R - 4, R)):
+        for b in (slice(0, 4), slice(C - 4, C)):
+            mask[a, b] = False
+    assert same(G[:, mask], ref[:, mask])
+
+
+@pytest.mark.parametrize("order", [3, 2])
+def test_double_mach_reflection_steps(oracle, order):
+    """The DMR workload with moving boundaries (stage-time ghost fill), Roe +
+    positivity fallback, WENO5-char: device vs oracle, whole array, through the
+    clipped driver loop."""
+    c, U0 = cases.dmr(96, 24, time_order=order)
+    U = U0.copy()
+    t_ref, dts = oracle.advance_to(c, U, 0.01, 1000)
+    with Solver(c) as s:
+        s.set_state(U0)
+        t, n = s.advance_to(0.01, 1000)
+        G = s.get_state()
+        assert s.get_time() == t
+        assert (s.sync() & ~4) == 0
+    assert n == len(dts) and t == t_ref and n > 5
+    assert same(G, U), first_diff(G, U)
+
+
 # ------------------------------------------------------------ full-size cases
 @pytest.mark.slow
 def test_lax_liu_2048_prefix(oracle):
