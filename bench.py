@@ -167,15 +167,17 @@ def cpu_baseline(n, steps, cores):
                       f"(dt + 3 stages, IO excluded), {secs:.2f} s"}
 
 
-def run_batch(steps, warmup):
+def run_batch(steps, warmup, precision=1):
     """The batching form on the seeded 64-problem four-quadrant workload (sizes 256..1536,
-    HLLC/HLL/Roe per problem, WENO5-char, RK3, CFL 0.5): autotune, then time `steps`
-    steps of the whole batch (every problem is still active: the smallest needs ~400
-    steps to reach t = 0.25)."""
+    HLLC/HLL/Roe per problem, WENO5-char, RK3, CFL 0.5), FP32 flux + FP64 state
+    (precision 1) or FP64: autotune, then time `steps` steps of the whole batch (every
+    problem is still active: the smallest needs ~400 steps to reach t = 0.25)."""
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.capi import Batch
     ev = HipEvents()
     cfgs, states = cases.batch_case(64)
+    for c in cfgs:
+        c.precision = precision
     cells = sum(c.nx * c.ny for c in cfgs)
     with Batch(cfgs) as b:
         for k, U in enumerate(states):
@@ -196,9 +198,9 @@ def run_batch(steps, warmup):
     value = cells * steps / (ms / 1e3) / 1e6
     return {"value": round(value, 3), "unit": "Mcell-updates/s", "ms_per_step": round(ms / steps, 4),
             "workload": "64 four-quadrant problems, n in {256..1536}, seed 20240611",
-            "cells": cells, "form": "fp64", "tuning": tuned,
-            "problems_active_throughout": active, "fault_bits": sorted(set(int(x) for x in f)),
-            "note": "the mixed-precision batch form is switched off"}
+            "cells": cells, "form": "mixed" if precision else "fp64", "tuning": tuned,
+            "problems_active_throughout": active,
+            "problems_with_fault_bits": {str(b): int((f == b).sum()) for b in sorted(set(int(x) for x in f))}}
 
 
 def run_dmr(steps, warmup):
@@ -253,31 +255,59 @@ def noh_metrics(states, n, gamma, t_end=0.6):
 
 
 def noh_full(n, t_end=0.6):
-    """Both forms to t_end on the Noh workload: L1 / L-inf of rho and p (mixed vs FP64,
-    interior) and the wall-heating density deficit near the origin.  The exact 2-D Noh
-    solution has rho = 16 behind the shock (r < t/3) for gamma = 5/3, so the deficit is
-    1 - rho/16 at the corner cell and over r < 0.05, with the plateau 0.1 < r < 0.15 as
-    the shocked-state reference."""
+    """Both forms to t_end on the Noh workload, with the accuracy report of
+    :func:`noh_metrics`.  Four runs: FP64 (its dt sequence recorded), mixed as it runs
+    (its own dts), mixed replaying the FP64 dts (the arithmetic difference alone), and
+    FP64 from a state perturbed by 1e-12 relative (the problem's own sensitivity to
+    round-off, the yardstick for the other two)."""
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.capi import Solver
     from overheatingproblem_amd.config import PRECISION_MIXED
     cfg, U0 = cases.noh(n, positivity=True)
-    res = {}
-    for name in ("fp64", "mixed"):
-        c = cfg.copy()
-        if name == "mixed":
-            c.precision = PRECISION_MIXED
+    mx = cfg.copy()
+    mx.precision = PRECISION_MIXED
+    rng = np.random.default_rng(12)
+    Up = U0.copy()
+    Up[0] *= 1.0 + 1e-12 * rng.uniform(-1.0, 1.0, size=Up[0].shape)
+
+    def run(c, U, dts=None, record=False):
         t0 = time.perf_counter()
+        used = []
         with Solver(c) as s:
-            s.set_state(U0)
-            t, steps = s.advance_to(t_end, 10 ** 7)
-            U = s.get_state()
-            faults = s.sync()
-        res[name] = {"U": U, "t": t, "steps": steps, "faults": faults,
-                     "wall_s": time.perf_counter() - t0}
-        log(f"bench: noh {n}^2 {name} to t={t} in {steps} steps, {res[name]['wall_s']:.1f} s")
+            s.set_state(U)
+            if dts is None and not record:
+                t, steps = s.advance_to(t_end, 10 ** 7)
+            else:
+                t, steps = 0.0, 0
+                while (t < t_end) if dts is None else (steps < len(dts)):
+                    if dts is None:  # the driver loop, src/RP_FVM.c:179-191
+                        dt = s.get_delta_t()
+                        if t + dt > t_end:
+                            dt = t_end - t
+                        if t + dt >= t_end:
+                            dt = t_end - t
+                    else:
+                        dt = dts[steps]
+                    used.append(dt)
+                    s.time_advance(dt)
+                    t = t + dt
+                    steps += 1
+            out = {"U": s.get_state(), "t": t, "steps": steps, "faults": s.sync(),
+                   "wall_s": time.perf_counter() - t0, "dts": used}
+        log(f"bench: noh {n}^2 to t={t} in {steps} steps, {out['wall_s']:.1f} s")
+        return out
+
+    res = {"fp64": run(cfg, U0, record=True)}
+    res["mixed"] = run(mx, U0)
+    res["mixed_replay"] = run(mx, U0, dts=res["fp64"]["dts"])
+    res["fp64_perturbed"] = run(cfg, Up)
     out = {"workload": f"noh-{n}x{n}", "t_end": t_end}
-    out.update(noh_metrics({k: v["U"] for k, v in res.items()}, n, cfg.gamma))
+    m = noh_metrics({"fp64": res["fp64"]["U"], "mixed": res["mixed"]["U"]}, n, cfg.gamma, t_end)
+    out.update(m)
+    for other in ("mixed_replay", "fp64_perturbed"):
+        mm = noh_metrics({"fp64": res["fp64"]["U"], "mixed": res[other]["U"]}, n, cfg.gamma, t_end)
+        out[other] = mm["mixed"]
+        out[other + "_vs_fp64"] = mm["mixed_vs_fp64"]
     for k, v in res.items():
         out[k].update({"steps": v["steps"], "faults": v["faults"], "wall_s": round(v["wall_s"], 2),
                        "t": v["t"]})
@@ -400,7 +430,7 @@ def main():
     if not args.no_batch:
         log("bench: batch form (64 uneven problems) ...")
         try:
-            out["batch"] = run_batch(10, 2)
+            out["batch"] = {"mixed": run_batch(10, 2, 1), "fp64": run_batch(10, 2, 0)}
         except Exception as e:  # the headline figure stands without it
             out["batch"] = {"value": None, "error": str(e)}
     if not args.no_cpu:

@@ -151,12 +151,6 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         for (int s = 0; s < 4; s++)
             if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
                 return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);
-        // The mixed-precision batch stage (k_batch_stage<..., MIXED> + k_batch_fix) did
-        // not complete its first step on the GPU and the cause is not yet identified;
-        // it stays switched off until it is.
-        if (c.precision == RPFVM_PRECISION_MIXED)
-            return fail(RPFVM_ERR_UNSUPPORTED,
-                        "problem %d: the mixed-precision batch form is disabled (FP64 only)", k);
         if (c.nx < 2 * GC || c.ny < 2 * GC || c.nx >= (1 << 20) - 2 * GC || c.ny >= (1 << 20) - 2 * GC)
             return fail(RPFVM_ERR_INVALID, "problem %d: nx, ny must be in 8 .. 2^20 - 9", k);
         if ((c.characteristic != 0) != (cfgs[0].characteristic != 0) ||

@@ -237,10 +237,14 @@ __global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int 
     const int nact = a.sched->nact, total = a.sched->total;
     unsigned* queue = &a.sched->queue[stage];
     __shared__ double ring[STAGING == STAGING_LDS ? 6 : 1][4][kRingCols];
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = (int)atomicAdd(queue, 1u);
-        item = uniform(__shfl(item, 0, 64));
+    // Work fetch without a lane-dependent branch: every lane takes part in the atomic,
+    // lane 0 adding 1 and the others 0, and the wavefront takes lane 0's ticket.  (An
+    // `if (lane == 0) atomicAdd` + shuffle let the compiler lower this loop as a
+    // divergent one whose latch re-entered the strip body without a new fetch.)  The
+    // trip count is bounded by the number of work items as well.
+    for (int trip = 0; trip <= total; trip++) {
+        const unsigned ticket = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const int item = uniform((int)ticket);
         if (item >= total) break;
         const int ai = uniform(owner(a, nact, item));
         const int k = uniform(a.act_prob[ai]);
@@ -576,8 +580,10 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
         hipLaunchKernelGGL((k_batch_stage<ORDER, CHAR, MIXED, STAGING_DIRECT>), dim3(l.grid),
                            dim3(64), 0, s, a, stage, kind, yi, oi, last);
     if constexpr (MIXED)
-        hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(64), dim3(256), 0, s, a, stage, kind, yi,
-                           oi, last);
+        // 1024 workgroups: at one wavefront per SIMD (its FP64 fallback path needs 320
+        // registers) that covers every SIMD of the chip
+        hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(1024), dim3(256), 0, s, a, stage, kind,
+                           yi, oi, last);
 }
 
 }  // namespace

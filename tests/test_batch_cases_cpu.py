@@ -45,9 +45,9 @@ def test_unstable_seed_is_detected(oracle):
 def test_batch_rejects_what_it_does_not_run():
     """Configuration checks happen before any device work."""
     from overheatingproblem_amd.capi import Batch, RpfvmError
-    from overheatingproblem_amd.config import PRECISION_MIXED
-    cfgs, _ = small_batch(PRECISION_MIXED, True)
-    with pytest.raises(RpfvmError, match="mixed-precision batch form is disabled"):
+    cfgs, _ = small_batch(0, True)
+    cfgs[2].positivity = 1
+    with pytest.raises(RpfvmError, match="no positivity safeguard"):
         Batch(cfgs)
     cfgs, _ = small_batch(0, True)
     cfgs[2].recon_order = 3

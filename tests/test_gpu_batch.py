@@ -18,8 +18,6 @@ from overheatingproblem_amd.config import (BC_FIXED_VALUE as BC_FIXED, BC_INFLOW
 
 pytestmark = pytest.mark.gpu
 GC = 4
-MIXED_OFF = ("the mixed-precision batch form is switched off: its first step did not complete "
-             "on the GPU (cause not identified)")
 
 
 def single(cfg, U0, t_end):
@@ -59,7 +57,7 @@ def small_batch(precision=0, characteristic=True, seed=21):
     return cfgs, states
 
 
-@pytest.mark.parametrize("precision", [0, pytest.param(PRECISION_MIXED, marks=pytest.mark.skip(reason=MIXED_OFF))])
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 @pytest.mark.parametrize("characteristic", [True, False])
 def test_batch_matches_single_problems(precision, characteristic):
     cfgs, states = small_batch(precision, characteristic)
@@ -152,7 +150,7 @@ def test_batch_rk_orders():
             assert np.array_equal(interior(got[k]), interior(ref)), (order, k)
 
 
-@pytest.mark.parametrize("precision", [0, pytest.param(PRECISION_MIXED, marks=pytest.mark.skip(reason=MIXED_OFF))])
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 def test_batch_seeded_workload_prefix(precision):
     """The first problems of the seeded 64-problem workload (sizes 256..1536), a few
     steps: identical to the single-problem path."""
