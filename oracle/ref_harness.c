@@ -193,6 +193,9 @@ int main(int argc, char **argv) {
                 case 1: HLL_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
                 case 2: HLLC_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
                 case 3: Roe_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
+                case 11: HLLHC_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
+                case 22: HLLCHC_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
+                case 33: RoeHC_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
                 default: LLF_Flux(dir, var, rows, cols, GhostCell, Ls, Rs, Fl); break;
             }
             write_doubles(out, (double *)Fl, nU);

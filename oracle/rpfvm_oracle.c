@@ -520,8 +520,20 @@ static int hlle_(double g, int dir, const double cl[4], const double cr[4], doub
     return 0;
 }
 
-/* Roe with a no-op entropy fix: include/scheme.h:1018-1322. */
+/* Roe with a no-op entropy fix: include/scheme.h:1018-1322.  With hc set it is
+ * RoeHC (include/scheme.h:2793-3110): the wave-strength reciprocals carry eps = 1e-12,
+ * a face whose Roe density sqrt(rho_L) sqrt(rho_R) is NaN is skipped (:2936-2939; here:
+ * NaN flux, return 1), and the energy flux loses 0.5 |q~ + c~| (e_R - e_L) rho~ with
+ * e = p / (rho (gamma - 1)). */
+static int roe_core_(double g, int dir, const double cl[4], const double cr[4], double F[4],
+                     int hc);
+
 static int roe_(double g, int dir, const double cl[4], const double cr[4], double F[4]) {
+    return roe_core_(g, dir, cl, cr, F, 0);
+}
+
+static int roe_core_(double g, int dir, const double cl[4], const double cr[4], double F[4],
+                     int hc) {
     const double gm1 = g - 1.0;
     const double gr = g / gm1;
     side_t L, R;
@@ -545,8 +557,8 @@ static int roe_(double g, int dir, const double cl[4], const double cr[4], doubl
     double lam[4] = {fabs(nbar - cbar), fabs(nbar), fabs(nbar), fabs(nbar + cbar)};
     double dr = R.r - L.r, du = R.u - L.u, dv = R.v - L.v, dp = R.p - L.p;
     double cs = cbar * cbar;
-    double i2cs = 1.0 / (2.0 * cs);
-    double ics = 1.0 / cs;
+    double i2cs = hc ? 1.0 / (2.0 * cs + 1e-12) : 1.0 / (2.0 * cs);
+    double ics = hc ? 1.0 / (cs + 1e-12) : 1.0 / cs;
     double dn = dir == 1 ? du : dv, dt = dir == 1 ? dv : du;
     double al[4] = {(dp - rbar * cbar * dn) * i2cs, dr - dp * ics, rbar * dt,
                     (dp + rbar * cbar * dn) * i2cs};
@@ -568,6 +580,127 @@ static int roe_(double g, int dir, const double cl[4], const double cr[4], doubl
     for (int w = 0; w < 4; w++) {
         double term = 0.5 * lam[w] * al[w];
         for (int k = 0; k < 4; k++) F[k] -= term * ev[w][k];
+    }
+    if (hc) {
+        if (isnan(rbar)) {
+            for (int k = 0; k < 4; k++) F[k] = NAN;
+            return 1;
+        }
+        const double igm1 = 1.0 / gm1;
+        double eL = L.ir * L.p * igm1, eR = R.ir * R.p * igm1;
+        F[3] = F[3] - 0.5 * lam[3] * (eR - eL) * rbar;
+    }
+    return 0;
+}
+
+/* HLL with the heat-capacity energy term (HLLHC): include/scheme.h:1860-2137.  Sound
+ * speed sqrt(gamma p / rho), Einfeldt speeds with eta = 0.5 sqrt(rho_L rho_R) / (sum)^2
+ * formed through the reciprocal of the sum; a NaN Einfeldt speed stays NaN on x faces
+ * (the fallback at :1948-1953 is commented out) and falls back to the Roe sound speed
+ * on y faces (:2073-2076).  The star-region energy flux adds
+ * (s_L s_R / (s_R - s_L)) (e_R - e_L) rho_HLL, rho_HLL the HLL mean density. */
+static int hllhc_(double g, int dir, const double cl[4], const double cr[4], double F[4]) {
+    const double gm1 = g - 1.0;
+    const double gr = g / gm1;
+    const double igm1 = 1.0 / gm1;
+    side_t L, R;
+    side_(g, cl, &L);
+    side_(g, cr, &R);
+    double aL = sqrt(g * L.p * L.ir);
+    double aR = sqrt(g * R.p * R.ir);
+    double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double FL[4], FR[4];
+    phys_flux_(dir, &L, HL, FL);
+    phys_flux_(dir, &R, HR, FR);
+    double sL = sqrt(L.r), sR = sqrt(R.r);
+    double ss = sL + sR;
+    double iss = 1.0 / ss;
+    double ubar = (sL * L.u + sR * R.u) * iss;
+    double vbar = (sL * L.v + sR * R.v) * iss;
+    double Hbar = (sL * HL + sR * HR) * iss;
+    double eta = 0.5 * sL * sR * iss * iss;
+    double qL = dir == 1 ? L.u : L.v, qR = dir == 1 ? R.u : R.v;
+    double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    if (dir == 2 && isnan(d)) d = sqrt(gm1 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)));
+    double nbar = dir == 1 ? ubar : vbar;
+    double sl = nbar - d, sr = nbar + d;
+    double rhll = (sr * R.r - sl * L.r + FL[0] - FR[0]) / (sr - sl);
+    double eL = L.p * L.ir * igm1, eR = R.p * R.ir * igm1;
+    if (sl >= 0.0) {
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sr <= 0.0) {
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    } else {
+        double isd = 1.0 / (sr - sl);
+        double slsr = sl * sr;
+        for (int k = 0; k < 3; k++) F[k] = (sr * FL[k] - sl * FR[k] + slsr * (cr[k] - cl[k])) * isd;
+        F[3] = (sr * FL[3] - sl * FR[3] + slsr * (cr[3] - cl[3])) * isd + (slsr * isd) * (eR - eL) * rhll;
+    }
+    return 0;
+}
+
+/* HLLC with the heat-capacity energy term (HLLCHC): include/scheme.h:2153-2478.  Sound
+ * speed from the enthalpy, eta through (sum)^2 in both directions, no NaN fallback.
+ * Branch order as the reference: s_L >= 0; s_L < 0 and s* >= 0; s* < 0 and s_R > 0;
+ * otherwise the right flux (so NaN speeds take the right flux).  The star-state energy
+ * fluxes add (s_L s_R / (s_R - s_L)) (e_R - e_L) times the star density. */
+static int hllchc_(double g, int dir, const double cl[4], const double cr[4], double F[4]) {
+    const double gm1 = g - 1.0;
+    const double gr = g / gm1;
+    const double igm1 = 1.0 / gm1;
+    side_t L, R;
+    side_(g, cl, &L);
+    side_(g, cr, &R);
+    double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
+    double aR = sqrt((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
+    double FL[4], FR[4];
+    phys_flux_(dir, &L, HL, FL);
+    phys_flux_(dir, &R, HR, FR);
+    double sL = sqrt(L.r), sR = sqrt(R.r);
+    double ss = sL + sR;
+    double iss = 1.0 / ss;
+    double nbar = dir == 1 ? (sL * L.u + sR * R.u) * iss : (sL * L.v + sR * R.v) * iss;
+    double eta = 0.5 * sL * sR / (ss * ss);
+    double qL = dir == 1 ? L.u : L.v, qR = dir == 1 ? R.u : R.v;
+    double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    double sl = nbar - d, sr = nbar + d;
+    double denom = L.r * (sl - qL) - R.r * (sr - qR);
+    double sstar = (R.p - L.p + L.r * qL * (sl - qL) - R.r * qR * (sr - qR)) / denom;
+    double mL = L.r * (sl - qL) / (sl - sstar);
+    double mR = R.r * (sr - qR) / (sr - sstar);
+    double feL = L.E * L.ir + (sstar - qL) * (sstar + L.p / (L.r * (sl - qL)));
+    double feR = R.E * R.ir + (sstar - qR) * (sstar + R.p / (R.r * (sr - qR)));
+    double eL = L.ir * L.p * igm1, eR = R.ir * R.p * igm1;
+    double isd = 1.0 / (sr - sl);
+    double slsr = sl * sr;
+    double ec = slsr * isd * (eR - eL);
+    if (sl >= 0.0) {
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sl < 0.0 && sstar >= 0.0) {
+        F[0] = FL[0] + sl * (mL - L.r);
+        if (dir == 1) {
+            F[1] = FL[1] + sl * (mL * sstar - L.ru);
+            F[2] = FL[2] + sl * (mL * L.v - L.rv);
+        } else {
+            F[1] = FL[1] + sl * (mL * L.u - L.ru);
+            F[2] = FL[2] + sl * (mL * sstar - L.rv);
+        }
+        F[3] = FL[3] + sl * (mL * feL - L.E) + ec * mL;
+    } else if (sstar < 0.0 && sr > 0.0) {
+        F[0] = FR[0] + sr * (mR - R.r);
+        if (dir == 1) {
+            F[1] = FR[1] + sr * (mR * sstar - R.ru);
+            F[2] = FR[2] + sr * (mR * R.v - R.rv);
+        } else {
+            F[1] = FR[1] + sr * (mR * R.u - R.ru);
+            F[2] = FR[2] + sr * (mR * sstar - R.rv);
+        }
+        F[3] = FR[3] + sr * (mR * feR - R.E) + ec * mR;
+    } else {
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
     }
     return 0;
 }
@@ -598,6 +731,9 @@ static int riemann_(const rpfvm_config *cfg, int dir, const double cl[4], const 
         case 1: return hll_(cfg->gamma, dir, cl, cr, F);
         case 2: return hllc_(cfg->gamma, dir, cl, cr, F);
         case 3: return roe_(cfg->gamma, dir, cl, cr, F);
+        case 11: return hllhc_(cfg->gamma, dir, cl, cr, F);
+        case 22: return hllchc_(cfg->gamma, dir, cl, cr, F);
+        case 33: return roe_core_(cfg->gamma, dir, cl, cr, F, 1);
         default: return llf_(cfg->gamma, dir, cl, cr, F);
     }
 }
@@ -607,10 +743,8 @@ static int check_cfg_(const rpfvm_config *c) {
     if (c->recon_order != 1 && c->recon_order != 2 && c->recon_order != 3 && c->recon_order != 5)
         return RPFVM_ERR_INVALID;
     if (c->time_order < 1 || c->time_order > 3) return RPFVM_ERR_INVALID;
-    /* Scheme 5 (RS_Marquina stub) and the heat-conduction ids 11/22/33 are not
-     * restated here. */
-    if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
-        return RPFVM_ERR_UNSUPPORTED;
+    /* Scheme 5 (the RS_Marquina stub) is not restated here. */
+    if (c->scheme == 5) return RPFVM_ERR_UNSUPPORTED;
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 6) return RPFVM_ERR_INVALID;
     /* the double-Mach-reflection sides exist on the bottom / top only */

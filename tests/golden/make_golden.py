@@ -114,6 +114,90 @@ def make_flux():
     return out
 
 
+def hc_face_pairs(rng, R, C, gamma):
+    """Face states for the heat-capacity solvers: physical pairs plus the faces that
+    reach their special branches -- supersonic pairs (all-left / all-right fluxes),
+    identical and resting states (s* on the branch boundary), near-vacuum densities,
+    negative pressure (NaN sound speed: the x/y HLLHC fallback asymmetry and HLLCHC's
+    right-flux default) and negative or zero density (RoeHC's skipped face)."""
+    shape = (R, C)
+    rl, ul, vl, pl = physical_pairs(rng, shape)
+    rr, ur, vr, pr = physical_pairs(rng, shape)
+    kind = rng.choice(8, size=shape, p=[0.40, 0.12, 0.12, 0.08, 0.08, 0.08, 0.07, 0.05])
+    near = kind == 1
+    jit = lambda a: a * (1.0 + 0.01 * rng.uniform(-1, 1, size=shape))
+    rr = np.where(near, jit(rl), rr)
+    ur = np.where(near, ul + 0.01 * rng.uniform(-1, 1, size=shape), ur)
+    vr = np.where(near, vl + 0.01 * rng.uniform(-1, 1, size=shape), vr)
+    pr = np.where(near, jit(pl), pr)
+    # strong, supersonic pairs: large ratios and speeds of either sign
+    strong = kind == 2
+    rr = np.where(strong, rl * 10.0 ** rng.uniform(-3, 3, size=shape), rr)
+    pr = np.where(strong, pl * 10.0 ** rng.uniform(-5, 5, size=shape), pr)
+    drift = rng.choice([-1.0, 1.0], size=shape) * rng.uniform(5, 40, size=shape)
+    ul = np.where(strong, ul + drift, ul)
+    ur = np.where(strong, ur + drift, ur)
+    vl = np.where(strong, vl + drift, vl)
+    vr = np.where(strong, vr + drift, vr)
+    # identical states; resting states with a pressure jump
+    same = kind == 3
+    rr, ur, vr, pr = (np.where(same, a, b) for a, b in ((rl, rr), (ul, ur), (vl, vr), (pl, pr)))
+    rest = kind == 4
+    ul, ur, vl, vr = (np.where(rest, 0.0, a) for a in (ul, ur, vl, vr))
+    # near vacuum on one side
+    vac = kind == 5
+    rl = np.where(vac & (rng.uniform(size=shape) < 0.5), 1e-10, rl)
+    rr = np.where(vac, np.where(rl == 1e-10, rr, 1e-9), rr)
+    # negative pressure on one or both sides
+    negp = kind == 6
+    pl = np.where(negp & (rng.uniform(size=shape) < 0.7), -rng.uniform(1e-3, 1.0, size=shape), pl)
+    pr = np.where(negp & (rng.uniform(size=shape) < 0.7), -rng.uniform(1e-3, 1.0, size=shape), pr)
+    Ls = cases.prim_to_cons(rl, ul, vl, pl, gamma)
+    Rs = cases.prim_to_cons(rr, ur, vr, pr, gamma)
+    # negative / zero density in the conserved state (momentum and energy kept)
+    negr = kind == 7
+    pick = rng.uniform(size=shape)
+    Ls[0] = np.where(negr & (pick < 0.4), -Ls[0], Ls[0])
+    Rs[0] = np.where(negr & (pick >= 0.4) & (pick < 0.8), -Rs[0], Rs[0])
+    Ls[0] = np.where(negr & (pick >= 0.8), 0.0, Ls[0])
+    return np.ascontiguousarray(Ls), np.ascontiguousarray(Rs)
+
+
+def make_flux_hc():
+    out = {}
+    nx, ny = 24, 20
+    R, C = nx + 8, ny + 8
+    rng = np.random.default_rng(2112)
+    for scheme in (11, 22, 33):
+        for gamma in (1.4, 5.0 / 3.0):
+            cfg = make_config(nx, ny, scheme=scheme, gamma=gamma)
+            Ls, Rs = hc_face_pairs(rng, R, C, gamma)
+            n = 4 * R * C
+            res = run(job_text("flux", cfg), [Ls, Rs], 2 * n)
+            key = f"s{scheme}_g{'53' if gamma > 1.5 else '14'}"
+            out[key + "_cfg"] = cfg_json(cfg)
+            out[key + "_L"] = Ls
+            out[key + "_R"] = Rs
+            out[key + "_F1"] = res[:n].reshape(4, R, C)
+            out[key + "_F2"] = res[n:].reshape(4, R, C)
+    # whole steps through the reference's solver switch (include/CFD_convection.h:99-107)
+    specs = []
+    for scheme in (11, 22, 33):
+        for char in (1, 0):
+            specs.append((f"ll_s{scheme}_c{char}",
+                          lambda s=scheme, c=char: ll_small(20, 16, scheme=s, characteristic=c), 5))
+        specs.append((f"sedov_s{scheme}", lambda s=scheme: cases.sedov(18, scheme=s), 4))
+    specs.append(("wave_s33", lambda: cases.smooth_wave(16, 12, scheme=33), 4))
+    for key, builder, steps in specs:
+        cfg, U = builder()
+        res = run(job_text("step", cfg, steps=steps), [U], steps + U.size)
+        out["step_" + key + "_cfg"] = cfg_json(cfg)
+        out["step_" + key + "_U0"] = U
+        out["step_" + key + "_dts"] = res[:steps]
+        out["step_" + key + "_U"] = res[steps:].reshape(cfg.shape)
+    return out
+
+
 def blocky_state(nx, ny, seed, gamma=1.4):
     """Smooth random state with a superposed oblique discontinuity."""
     U = cases.random_smooth_state(nx, ny, seed, gamma=gamma)
@@ -300,7 +384,8 @@ def main():
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
     makers = (("weno", make_weno), ("flux", make_flux), ("eigen", make_eigen),
               ("recon", make_recon), ("bc", make_bc), ("residual", make_residual),
-              ("steps", make_steps), ("dt", make_dt), ("dmrbc", make_dmrbc))
+              ("steps", make_steps), ("dt", make_dt), ("dmrbc", make_dmrbc),
+              ("flux_hc", make_flux_hc))
     only = set(sys.argv[1:])  # fixture names to (re)write; all when empty
     for name, fn in makers:
         if only and name not in only:
