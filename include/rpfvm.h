@@ -51,8 +51,13 @@ enum {
 };
 
 /* Riemann solver ids as the "scheme" control key, include/CFD_convection.h:89-117.
- * Any id that is not 1, 2 or 3 selects LLF, as the reference's default branch. */
-enum { RPFVM_SCHEME_HLL = 1, RPFVM_SCHEME_HLLC = 2, RPFVM_SCHEME_ROE = 3, RPFVM_SCHEME_LLF = 10 };
+ * Any other id except 5 selects LLF, as the reference's default branch.  The
+ * heat-capacity solvers 11, 22, 33 (HLLHC_Flux, HLLCHC_Flux, RoeHC_Flux,
+ * include/scheme.h:1860, 2153, 2793) run in the FP64 form only. */
+enum {
+    RPFVM_SCHEME_HLL = 1, RPFVM_SCHEME_HLLC = 2, RPFVM_SCHEME_ROE = 3, RPFVM_SCHEME_LLF = 10,
+    RPFVM_SCHEME_HLLHC = 11, RPFVM_SCHEME_HLLCHC = 22, RPFVM_SCHEME_ROEHC = 33
+};
 
 /* Arithmetic form of the step. */
 enum {
@@ -62,8 +67,9 @@ enum {
 
 /* Fault bits reported by rpfvm_get_faults (also returned by rpfvm_sync). */
 enum {
-    RPFVM_FAULT_HLL_BAD_STATE = 1,   /* HLL face with rho <= 0, NaN rho or NaN wave speed
-                                        (the reference prints and skips: scheme.h:427-436,500-503) */
+    RPFVM_FAULT_HLL_BAD_STATE = 1,   /* HLL face with rho <= 0, NaN rho or NaN wave speed,
+                                        or RoeHC face with a NaN Roe density (the reference
+                                        prints and skips: scheme.h:427-436,500-503,2936-2939) */
     RPFVM_FAULT_NONFINITE_DT = 2,    /* dt reduction produced a non-finite or non-positive dt */
     RPFVM_FAULT_MIXED_FALLBACK = 4,  /* mixed form recomputed at least one face in FP64
                                         (informational, not an error) */
@@ -154,6 +160,17 @@ int rpfvm_boundary_conditions(rpfvm_ctx *ctx);
  * L[4][R][C] (interior), F[4][R][C] (x faces i+1/2 at [i], i = 3..R-5) and
  * G[4][R][C] (y faces j+1/2 at [j]); any of them may be NULL.  Synchronous. */
 int rpfvm_space_operator(rpfvm_ctx *ctx, double *L, double *F, double *G);
+
+/* One Riemann solver alone, as the reference's per-solver flux calls
+ * HLL_Flux / HLLC_Flux / Roe_Flux / LLF_Flux / HLLHC_Flux / HLLCHC_Flux / RoeHC_Flux
+ * (dir, var, rows, cols, GC, L_state, R_state, flux) (include/scheme.h; dispatched at
+ * include/CFD_convection.h:89-117): the context's scheme on host face states
+ * Ls[4][R][C], Rs[4][R][C] for dir 1 (x faces i+1/2 at [i], i = 3..R-5, j = 4..C-5)
+ * or 2 (y faces), into host F[4][R][C]; other entries of F are left as they are on
+ * the device (the flux array of the context).  A face the reference skips gets NaN and
+ * sets RPFVM_FAULT_HLL_BAD_STATE.  FP64 contexts only; overwrites the stage arrays,
+ * not the state.  Synchronous. */
+int rpfvm_riemann(rpfvm_ctx *ctx, int dir, const double *Ls, const double *Rs, double *F);
 
 /* Time-controlled loop of the reference driver (src/RP_FVM.c:179-191), on the
  * device: repeat { dt = Get_Delta_T_2D; clip to t_stop; step } until *t reaches

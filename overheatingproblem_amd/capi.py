@@ -31,7 +31,7 @@ LIB_PATH = os.path.join(HERE, "lib", "librpfvm.so")
 EXPORTS = (
     "rpfvm_config_default", "rpfvm_create", "rpfvm_destroy", "rpfvm_set_state",
     "rpfvm_get_state", "rpfvm_get_delta_t", "rpfvm_time_advance", "rpfvm_boundary_conditions",
-    "rpfvm_space_operator", "rpfvm_advance_to", "rpfvm_sync", "rpfvm_get_faults",
+    "rpfvm_space_operator", "rpfvm_riemann", "rpfvm_advance_to", "rpfvm_sync", "rpfvm_get_faults",
     "rpfvm_device_state", "rpfvm_stream", "rpfvm_set_kernel_timing", "rpfvm_kernel_times",
     "rpfvm_last_error", "rpfvm_version", "rpfvm_set_time", "rpfvm_get_time",
     "rpfvm_batch_create", "rpfvm_batch_destroy", "rpfvm_batch_set_state", "rpfvm_batch_get_state",
@@ -80,6 +80,7 @@ def lib():
     L.rpfvm_time_advance.argtypes = [vp, ctypes.c_double]
     L.rpfvm_boundary_conditions.argtypes = [vp]
     L.rpfvm_space_operator.argtypes = [vp, dp, dp, dp]
+    L.rpfvm_riemann.argtypes = [vp, ctypes.c_int, dp, dp, dp]
     L.rpfvm_advance_to.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp, ip]
     L.rpfvm_sync.argtypes = [vp]
     L.rpfvm_get_faults.argtypes = [vp, ip]
@@ -184,6 +185,18 @@ class Solver:
         G = np.zeros(self.shape)
         _check(lib().rpfvm_space_operator(self._ctx, _dptr(L), _dptr(F), _dptr(G)))
         return L, F, G
+
+    def riemann(self, direction: int, Ls: np.ndarray, Rs: np.ndarray) -> np.ndarray:
+        """The configured solver alone on host face states (rpfvm_riemann): the face
+        fluxes of ``direction`` (1: x, 2: y) at the faces' left cells (entries off the
+        face range are unspecified)."""
+        Ls = np.ascontiguousarray(Ls, dtype=np.float64)
+        Rs = np.ascontiguousarray(Rs, dtype=np.float64)
+        if Ls.shape != self.shape or Rs.shape != self.shape:
+            raise ValueError(f"face states must have shape {self.shape}")
+        F = np.zeros(self.shape)
+        _check(lib().rpfvm_riemann(self._ctx, int(direction), _dptr(Ls), _dptr(Rs), _dptr(F)))
+        return F
 
     def set_time(self, t: float) -> None:
         """Clock read by the time-dependent boundary sides (rpfvm_set_time)."""

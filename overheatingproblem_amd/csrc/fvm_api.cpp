@@ -117,10 +117,15 @@ int fvm::api::validate(const rpfvm_config* c) {
         return fail(RPFVM_ERR_INVALID, "time_order must be 1, 2 or 3");
     if (c->recon_order != 1 && c->recon_order != 2 && c->recon_order != 3 && c->recon_order != 5)
         return fail(RPFVM_ERR_INVALID, "recon_order must be 1, 2, 3 or 5");
-    if (c->scheme == 5 || c->scheme == 11 || c->scheme == 22 || c->scheme == 33)
-        return fail(RPFVM_ERR_UNSUPPORTED, "scheme %d is not implemented on the device", c->scheme);
+    if (c->scheme == 5)
+        return fail(RPFVM_ERR_UNSUPPORTED, "scheme 5 (the Marquina stub) is not implemented");
     if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
         return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
+    if (c->precision == RPFVM_PRECISION_MIXED &&
+        (c->scheme == RPFVM_SCHEME_HLLHC || c->scheme == RPFVM_SCHEME_HLLCHC ||
+         c->scheme == RPFVM_SCHEME_ROEHC))
+        return fail(RPFVM_ERR_UNSUPPORTED,
+                    "the heat-capacity schemes 11, 22, 33 run in the FP64 form only");
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 6) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
     if (c->bc[0] >= RPFVM_BC_DMR_WALL || c->bc[1] >= RPFVM_BC_DMR_WALL ||
@@ -426,6 +431,23 @@ int rpfvm_get_delta_t(rpfvm_ctx* x, double* dt) {
     HIPCHK(hipMemcpyAsync(x->host_scal, x->scal, sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     *dt = x->host_scal[0];
+    return 0;
+}
+
+int rpfvm_riemann(rpfvm_ctx* x, int dir, const double* Ls, const double* Rs, double* F) {
+    if (!x || !Ls || !Rs || !F) return fail(RPFVM_ERR_INVALID, "null argument");
+    if (dir != 1 && dir != 2) return fail(RPFVM_ERR_INVALID, "dir must be 1 (x) or 2 (y)");
+    if (x->cfg.precision != RPFVM_PRECISION_FP64)
+        return fail(RPFVM_ERR_UNSUPPORTED, "rpfvm_riemann evaluates the FP64 solvers: FP64 contexts only");
+    // the face states travel through the stage arrays U1 / U2 (the state U is untouched)
+    int e = copy_in(x, x->U1, Ls);
+    if (!e) e = copy_in(x, x->U2, Rs);
+    if (e) return e;
+    const int solver = x->cfg.scheme;
+    launch_riemann(dir, solver, x->U1, x->U2, x->F, x->g, x->ph, x->faults, x->stream);
+    HIPCHK(hipGetLastError());
+    if ((e = copy_out(x, F, x->F))) return e;
+    HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
 }
 

@@ -148,6 +148,10 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: the batch form runs recon_order 5", k);
         if (c.positivity)
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: no positivity safeguard in a batch", k);
+        if (c.scheme == RPFVM_SCHEME_HLLHC || c.scheme == RPFVM_SCHEME_HLLCHC ||
+            c.scheme == RPFVM_SCHEME_ROEHC)
+            return fail(RPFVM_ERR_UNSUPPORTED,
+                        "problem %d: the heat-capacity schemes are not batched", k);
         for (int s = 0; s < 4; s++)
             if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
                 return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);

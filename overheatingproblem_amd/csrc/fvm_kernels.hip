@@ -112,6 +112,29 @@ __global__ FLUX_BOUNDS void k_flux(const double* __restrict__ Y, double* __restr
     for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
 }
 
+// Solver alone on given face states, as the reference's per-solver calls
+// *_Flux(dir, var, rows, cols, GC, L_state, R_state, flux) (include/scheme.h), over the
+// face ranges of k_flux.
+template <int DIR, int SOLVER>
+__global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ Ls,
+                                                 const double* __restrict__ Rs,
+                                                 double* __restrict__ F, Geom g, Phys ph,
+                                                 unsigned* __restrict__ faults) {
+    const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
+    const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.R - GC || j >= g.C - GC) return;
+    const size_t c0 = (size_t)i * g.ld + j;
+    double cl[4], cr[4], f[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        cl[v] = Ls[(size_t)v * g.plane + c0];
+        cr[v] = Rs[(size_t)v * g.plane + c0];
+    }
+    if (riemann<DIR, SOLVER>(make_consts(ph), cl, cr, f)) atomicOr(faults, 1u);
+#pragma unroll
+    for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+}
+
 // FP64 y faces fused with the cell update (fvm_fused.hpp).
 template <int ORDER, bool CHAR, int SOLVER>
 struct FaceOp64 {
@@ -382,6 +405,9 @@ static FluxPair pick2(int solver) {
         case SOLVER_HLL: return pick3<ORDER, CHAR, SOLVER_HLL>();
         case SOLVER_HLLC: return pick3<ORDER, CHAR, SOLVER_HLLC>();
         case SOLVER_ROE: return pick3<ORDER, CHAR, SOLVER_ROE>();
+        case SOLVER_HLLHC: return pick3<ORDER, CHAR, SOLVER_HLLHC>();
+        case SOLVER_HLLCHC: return pick3<ORDER, CHAR, SOLVER_HLLCHC>();
+        case SOLVER_ROEHC: return pick3<ORDER, CHAR, SOLVER_ROEHC>();
         default: return pick3<ORDER, CHAR, SOLVER_LLF>();
     }
 }
@@ -398,6 +424,35 @@ FluxPair select_flux(int order, bool characteristic, int solver) {
         case 3: return pick1<3>(characteristic, solver);
         default: return pick1<5>(characteristic, solver);
     }
+}
+
+template <int DIR, int SOLVER>
+static void launch_riemann_t(const double* Ls, const double* Rs, double* F, const Geom& g,
+                             const Phys& ph, unsigned* faults, hipStream_t s) {
+    const int nj = DIR == 1 ? g.C - 2 * GC : g.C - 2 * GC + 1;
+    const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
+    hipLaunchKernelGGL((k_riemann<DIR, SOLVER>), dim3((nj + 63) / 64, (ni + 3) / 4), dim3(64, 4),
+                       0, s, Ls, Rs, F, g, ph, faults);
+}
+
+template <int DIR>
+static void launch_riemann_d(int solver, const double* Ls, const double* Rs, double* F,
+                             const Geom& g, const Phys& ph, unsigned* faults, hipStream_t s) {
+    switch (solver) {
+        case SOLVER_HLL: return launch_riemann_t<DIR, SOLVER_HLL>(Ls, Rs, F, g, ph, faults, s);
+        case SOLVER_HLLC: return launch_riemann_t<DIR, SOLVER_HLLC>(Ls, Rs, F, g, ph, faults, s);
+        case SOLVER_ROE: return launch_riemann_t<DIR, SOLVER_ROE>(Ls, Rs, F, g, ph, faults, s);
+        case SOLVER_HLLHC: return launch_riemann_t<DIR, SOLVER_HLLHC>(Ls, Rs, F, g, ph, faults, s);
+        case SOLVER_HLLCHC: return launch_riemann_t<DIR, SOLVER_HLLCHC>(Ls, Rs, F, g, ph, faults, s);
+        case SOLVER_ROEHC: return launch_riemann_t<DIR, SOLVER_ROEHC>(Ls, Rs, F, g, ph, faults, s);
+        default: return launch_riemann_t<DIR, SOLVER_LLF>(Ls, Rs, F, g, ph, faults, s);
+    }
+}
+
+void launch_riemann(int dir, int solver, const double* Ls, const double* Rs, double* F,
+                    const Geom& g, const Phys& ph, unsigned* faults, hipStream_t s) {
+    if (dir == 1) launch_riemann_d<1>(solver, Ls, Rs, F, g, ph, faults, s);
+    else launch_riemann_d<2>(solver, Ls, Rs, F, g, ph, faults, s);
 }
 
 void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st) {

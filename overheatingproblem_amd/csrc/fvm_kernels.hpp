@@ -86,6 +86,9 @@ struct FluxPair {
 
 FluxPair select_flux(int order, bool characteristic, int solver);
 FluxPair select_flux_mixed(int order, bool characteristic, int solver);
+// FP64 solver `solver` on face states Ls/Rs into F, faces of direction dir (1 or 2).
+void launch_riemann(int dir, int solver, const double* Ls, const double* Rs, double* F,
+                    const Geom& g, const Phys& ph, unsigned* faults, hipStream_t s);
 void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st);
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
                    const double* G, const Geom& g, const Phys& ph, double dt, int kind,

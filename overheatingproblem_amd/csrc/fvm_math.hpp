@@ -543,8 +543,11 @@ __device__ __forceinline__ int hlle(const Consts& k, const double cl[4], const d
     return 0;
 }
 
-// Roe with the reference's no-op entropy fix, include/scheme.h:1018-1322.
-template <int DIR>
+// Roe with the reference's no-op entropy fix, include/scheme.h:1018-1322.  With HC it
+// is RoeHC (include/scheme.h:2793-3110): eps = 1e-12 in the wave-strength reciprocals,
+// a NaN Roe density skips the face (:2936-2939; here NaN and a fault bit), and the
+// energy flux loses 0.5 |q~ + c~| (e_R - e_L) rho~, e = p / (rho (gamma - 1)).
+template <int DIR, bool HC = false>
 __device__ __forceinline__ int roe(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
     const double g = k.g, gm1 = k.gm1, gr = k.gr;
     const Side L = make_side(k, cl), R = make_side(k, cr);
@@ -566,8 +569,8 @@ __device__ __forceinline__ int roe(const Consts& k, const double cl[4], const do
     const double lam[4] = {fabs(nbar - cbar), fabs(nbar), fabs(nbar), fabs(nbar + cbar)};
     const double dr = R.r - L.r, du = R.u - L.u, dv = R.v - L.v, dp = R.p - L.p;
     const double cs = cbar * cbar;
-    const double i2cs = 1.0 / (2.0 * cs);
-    const double ics = 1.0 / cs;
+    const double i2cs = HC ? 1.0 / (2.0 * cs + 1e-12) : 1.0 / (2.0 * cs);
+    const double ics = HC ? 1.0 / (cs + 1e-12) : 1.0 / cs;
     const double dn = DIR == 1 ? du : dv, dtg = DIR == 1 ? dv : du;
     const double al[4] = {(dp - rbar * cbar * dn) * i2cs, dr - dp * ics, rbar * dtg,
                           (dp + rbar * cbar * dn) * i2cs};
@@ -591,6 +594,135 @@ __device__ __forceinline__ int roe(const Consts& k, const double cl[4], const do
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] -= term * ev[w][k];
     }
+    if constexpr (HC) {
+        if (isnan(rbar)) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) F[k] = __builtin_nan("");
+            return 1;
+        }
+        const double igm1 = 1.0 / gm1;
+        const double eL = L.ir * L.p * igm1, eR = R.ir * R.p * igm1;
+        F[3] = F[3] - 0.5 * lam[3] * (eR - eL) * rbar;
+    }
+    return 0;
+}
+
+// HLL with the heat-capacity energy term (HLLHC), include/scheme.h:1860-2137: sound
+// speed sqrt(gamma p / rho), eta through the reciprocal of sqrt(rho_L) + sqrt(rho_R)
+// squared, a NaN Einfeldt speed kept on x faces (the fallback is commented out there,
+// :1948-1953) and replaced by the Roe sound speed on y faces (:2073-2076); the
+// star-region energy flux adds (s_L s_R / (s_R - s_L)) (e_R - e_L) rho_HLL.
+template <int DIR>
+__device__ __forceinline__ int hllhc(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    const double g = k.g, gm1 = k.gm1, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
+    const double aL = sqrt(g * L.p * L.ir);
+    const double aR = sqrt(g * R.p * R.ir);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double ss = sL + sR;
+    const double iss = 1.0 / ss;
+    const double ubar = (sL * L.u + sR * R.u) * iss;
+    const double vbar = (sL * L.v + sR * R.v) * iss;
+    const double eta = 0.5 * sL * sR * iss * iss;
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    if constexpr (DIR == 2) {
+        if (isnan(d)) {
+            const double Hbar = (sL * HL + sR * HR) * iss;
+            d = sqrt(gm1 * (Hbar - 0.5 * (ubar * ubar + vbar * vbar)));
+        }
+    }
+    const double nbar = DIR == 1 ? ubar : vbar;
+    const double sl = nbar - d, sr = nbar + d;
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+    } else if (sr <= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    } else {
+        const double rhll = (sr * R.r - sl * L.r + FL[0] - FR[0]) / (sr - sl);
+        const double igm1 = 1.0 / gm1;
+        const double eL = L.p * L.ir * igm1, eR = R.p * R.ir * igm1;
+        const double isd = 1.0 / (sr - sl);
+        const double slsr = sl * sr;
+#pragma unroll
+        for (int k = 0; k < 3; k++) F[k] = (sr * FL[k] - sl * FR[k] + slsr * (cr[k] - cl[k])) * isd;
+        F[3] = (sr * FL[3] - sl * FR[3] + slsr * (cr[3] - cl[3])) * isd + (slsr * isd) * (eR - eL) * rhll;
+    }
+    return 0;
+}
+
+// HLLC with the heat-capacity energy term (HLLCHC), include/scheme.h:2153-2478: sound
+// speed from the enthalpy, eta through (sqrt(rho_L) + sqrt(rho_R))^2 in both
+// directions, no NaN fallback, and the reference's branch order (s_L >= 0; s_L < 0 and
+// s* >= 0; s* < 0 and s_R > 0; else the right flux, which NaN speeds also take).  The
+// star-state energy fluxes add (s_L s_R / (s_R - s_L)) (e_R - e_L) times the star
+// density.
+template <int DIR>
+__device__ __forceinline__ int hllchc(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    const double g = k.g, gm1 = k.gm1, gr = k.gr;
+    const Side L = make_side(k, cl), R = make_side(k, cr);
+    const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
+    const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
+    const double aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
+    const double aR = sqrt((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
+    double FL[4], FR[4];
+    phys_flux<DIR>(L, HL, FL);
+    phys_flux<DIR>(R, HR, FR);
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double ss = sL + sR;
+    const double iss = 1.0 / ss;
+    const double nbar = ((DIR == 1 ? sL * L.u + sR * R.u : sL * L.v + sR * R.v)) * iss;
+    const double eta = 0.5 * sL * sR / (ss * ss);
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    const double sl = nbar - d, sr = nbar + d;
+    if (sl >= 0.0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FL[k];
+        return 0;
+    }
+    const double denom = L.r * (sl - qL) - R.r * (sr - qR);
+    const double sstar = (R.p - L.p + L.r * qL * (sl - qL) - R.r * qR * (sr - qR)) / denom;
+    const double igm1 = 1.0 / gm1;
+    const double eL = L.ir * L.p * igm1, eR = R.ir * R.p * igm1;
+    const double isd = 1.0 / (sr - sl);
+    const double slsr = sl * sr;
+    const double ec = slsr * isd * (eR - eL);
+    if (sl < 0.0 && sstar >= 0.0) {
+        const double m = L.r * (sl - qL) / (sl - sstar);
+        const double e = L.E * L.ir + (sstar - qL) * (sstar + L.p / (L.r * (sl - qL)));
+        F[0] = FL[0] + sl * (m - L.r);
+        if constexpr (DIR == 1) {
+            F[1] = FL[1] + sl * (m * sstar - L.ru);
+            F[2] = FL[2] + sl * (m * L.v - L.rv);
+        } else {
+            F[1] = FL[1] + sl * (m * L.u - L.ru);
+            F[2] = FL[2] + sl * (m * sstar - L.rv);
+        }
+        F[3] = FL[3] + sl * (m * e - L.E) + ec * m;
+    } else if (sstar < 0.0 && sr > 0.0) {
+        const double m = R.r * (sr - qR) / (sr - sstar);
+        const double e = R.E * R.ir + (sstar - qR) * (sstar + R.p / (R.r * (sr - qR)));
+        F[0] = FR[0] + sr * (m - R.r);
+        if constexpr (DIR == 1) {
+            F[1] = FR[1] + sr * (m * sstar - R.ru);
+            F[2] = FR[2] + sr * (m * R.v - R.rv);
+        } else {
+            F[1] = FR[1] + sr * (m * R.u - R.ru);
+            F[2] = FR[2] + sr * (m * sstar - R.rv);
+        }
+        F[3] = FR[3] + sr * (m * e - R.E) + ec * m;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = FR[k];
+    }
     return 0;
 }
 
@@ -613,13 +745,19 @@ __device__ __forceinline__ int llf(const Consts& k, const double cl[4], const do
     return 0;
 }
 
-enum : int { SOLVER_HLL = 1, SOLVER_HLLC = 2, SOLVER_ROE = 3, SOLVER_LLF = 10 };
+enum : int {
+    SOLVER_HLL = 1, SOLVER_HLLC = 2, SOLVER_ROE = 3, SOLVER_LLF = 10,
+    SOLVER_HLLHC = 11, SOLVER_HLLCHC = 22, SOLVER_ROEHC = 33
+};
 
 template <int DIR, int SOLVER>
 __device__ __forceinline__ int riemann(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
     if constexpr (SOLVER == SOLVER_HLL) return hll<DIR>(k, cl, cr, F);
     else if constexpr (SOLVER == SOLVER_HLLC) return hllc<DIR>(k, cl, cr, F);
     else if constexpr (SOLVER == SOLVER_ROE) return roe<DIR>(k, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLHC) return hllhc<DIR>(k, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLCHC) return hllchc<DIR>(k, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_ROEHC) return roe<DIR, true>(k, cl, cr, F);
     else return llf<DIR>(k, cl, cr, F);
 }
 

@@ -612,11 +612,12 @@ int main(int argc, char **argv) {
     cfg.nx = c.nx;
     cfg.ny = c.ny;
     cfg.time_order = c.time_adm;
-    if (c.scheme == 11 || c.scheme == 22 || c.scheme == 33 || c.scheme == 5) {
-        fprintf(stderr, "scheme %d (%s) is not implemented on the GPU\n", c.scheme, scheme_name(c.scheme));
+    if (c.scheme == 5) {
+        fprintf(stderr, "scheme 5 (the Marquina stub) is not implemented on the GPU\n");
         return 2;
     }
-    cfg.scheme = (c.scheme == 1 || c.scheme == 2 || c.scheme == 3) ? c.scheme : RPFVM_SCHEME_LLF;
+    cfg.scheme = (c.scheme == 1 || c.scheme == 2 || c.scheme == 3 || c.scheme == 11 ||
+                  c.scheme == 22 || c.scheme == 33) ? c.scheme : RPFVM_SCHEME_LLF;
     cfg.recon_order = control_recon ? c.recon : 5;
     cfg.characteristic = control_recon ? c.characteriz : 0;
     cfg.source = c.source;

@@ -72,3 +72,21 @@ def test_bench_noh_metrics_on_oracle_states(oracle):
     out = bench.noh_metrics({"fp64": U, "mixed": U.copy()}, 32, cfg.gamma)
     assert out["mixed_vs_fp64"]["l1_rho"] == 0.0
     assert 0.0 < out["fp64"]["rho_plateau_0.1_0.15"] < 20.0
+
+
+@pytest.mark.parametrize("scheme,precision,message", [
+    (5, 0, "Marquina"),                      # the reference's stub solver
+    (11, 1, "FP64 form only"), (22, 1, "FP64 form only"), (33, 1, "FP64 form only"),
+])
+def test_create_rejects_unsupported_forms(lib, scheme, precision, message):
+    """Configuration checks run before any device call, so they are testable here."""
+    cfg = make_config(16, 16, scheme=scheme, precision=precision)
+    with pytest.raises(capi.RpfvmError, match=message):
+        capi.Solver(cfg)
+
+
+@pytest.mark.parametrize("scheme", [11, 22, 33])
+def test_batch_rejects_heat_capacity_schemes(lib, scheme):
+    cfgs = [make_config(16, 16, scheme=2), make_config(16, 16, scheme=scheme)]
+    with pytest.raises(capi.RpfvmError, match="problem 1: the heat-capacity schemes"):
+        capi.Batch(cfgs)
