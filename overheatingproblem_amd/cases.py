@@ -194,6 +194,27 @@ def smooth_wave(nx, ny, *, gamma=1.4, amp=0.2, t=0.0, **cfg_kw):
     return cfg, U
 
 
+def precision_wave(nx, *, gamma=1.4, **cfg_kw):
+    """Init_Precision (include/initialize.h:36-60, 1609-1629): on [0, 2 pi] x [0, 1] with
+    ny = 1, rho = 1 + 0.2 sin((i + 1/2) dx), u = 1, v = 0, p = 1 in the interior cells of
+    column GC (every other entry zero, as the reference leaves them), periodic in x and
+    outflow in y; exact density 1 + 0.2 sin(x - t), end time 2."""
+    R, C = nx + 2 * GHOST, 1 + 2 * GHOST
+    lx = 2.0 * math.pi
+    dx = lx / nx
+    rho = np.zeros((R, C))
+    u = np.zeros((R, C))
+    p = np.zeros((R, C))
+    for i in range(GHOST, R - GHOST):
+        rho[i, GHOST] = 1.0 + 0.2 * math.sin((i - GHOST + 0.5) * dx)
+        u[i, GHOST] = 1.0
+        p[i, GHOST] = 1.0
+    U = prim_to_cons(rho, u, np.zeros((R, C)), p, gamma)
+    cfg = make_config(nx, 1, dx=dx, dy=1.0, gamma=gamma,
+                      bc=(BC_PERIODICITY, BC_PERIODICITY, BC_OUTFLOW, BC_OUTFLOW), **cfg_kw)
+    return cfg, U
+
+
 def random_smooth_state(nx, ny, seed, *, gamma=1.4, amp=0.3, vel=1.0, modes=3):
     """A seeded smooth positive state (sum of random Fourier modes) on the full grid."""
     rng = np.random.default_rng(seed)

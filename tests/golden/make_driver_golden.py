@@ -5,7 +5,8 @@
 Runs oracle/_ref/rp_fvm (the reference's src/RP_FVM.c, compiled from its own sources)
 on small control files, one OpenMP thread (its 2-D Noh initialiser draws rand() inside
 a parallel loop), and stores every output file (.plt / .dat) it writes, by name, in
-tests/golden/driver.npz together with the control-file text (output_dir left out).
+tests/golden/driver.npz together with the control-file text (output_dir left out); for
+the Precision_Test runs, which write no files, the printed errors and error table.
 """
 import os
 import subprocess
@@ -36,6 +37,14 @@ CASES = {
     "backstep_rk1_steps": dict(TestCase="Backward_Step", L_nx=24, L_ny=8, scheme=2, Time_ADM=1,
                                Control_Compution=1, Control_output=100),
 }
+# Precision_Test runs (src/RP_FVM.c:974-1072): no output files; the golden is the
+# program's stdout from the first error line on (errors, orders, the error table)
+PRECISION_CASES = {
+    "precision_hllc": dict(TestCase="Precision_Test", L_nx=10, L_ny=1, scheme=2),
+    "precision_hllhc": dict(TestCase="Precision_Test", L_nx=10, L_ny=1, scheme=11),
+    "precision_roehc_rk2": dict(TestCase="Precision_Test", L_nx=10, L_ny=1, scheme=33, Time_ADM=2),
+}
+STDOUT_FROM = "L1 Error = "
 DEFAULTS = dict(Time_ADM=3, M_gamma=1.4, Source="false", Gravity=1.0, CFL=0.4, Tmax=1.0,
                 Control_Compution=0)
 
@@ -68,6 +77,18 @@ def main():
                 with open(os.path.join(out, fn)) as f:
                     data[f"{name}__{k}"] = np.array(f.read())
         print(name, files, flush=True)
+    for name, keys in PRECISION_CASES.items():
+        with tempfile.TemporaryDirectory() as td:
+            cf = os.path.join(td, "ControlFile.txt")
+            with open(cf, "w") as f:
+                f.write(control_text(keys, os.path.join(td, "out")))
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            r = subprocess.run([PROG, cf], cwd=td, env=env, check=True, capture_output=True,
+                               text=True, timeout=600)
+            out = r.stdout[r.stdout.index(STDOUT_FROM):]
+            data[name + "__control"] = np.array(control_text(keys, "@OUT@"))
+            data[name + "__stdout"] = np.array(out)
+        print(name, out.splitlines()[-3], flush=True)
     np.savez_compressed(OUT, **data)
     print(f"{OUT}: {os.path.getsize(OUT) / 1024:.1f} KiB")
 

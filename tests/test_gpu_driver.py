@@ -21,9 +21,9 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "driver.npz")
 
 
-def cases():
+def cases(kind="files"):
     d = np.load(GOLDEN, allow_pickle=False)
-    return sorted({k.split("__")[0] for k in d.files})
+    return sorted(k.split("__")[0] for k in d.files if k.endswith("__" + kind))
 
 
 def norm(text):
@@ -49,3 +49,24 @@ def test_driver_matches_reference_program(name, tmp_path):
             bad = [i for i, (a, b) in enumerate(zip(gl, rl)) if a != b]
             pytest.fail(f"{fn}: {len(bad)} lines differ, first at {bad[:1]}: "
                         f"{gl[bad[0]] if bad else ''!r} vs {rl[bad[0]] if bad else ''!r}")
+
+
+@pytest.mark.parametrize("name", cases("stdout"))
+def test_precision_harness_matches_reference_program(name, tmp_path):
+    """TestCase Precision_Test: seven grids to T = 2 on the GPU; the printed L1 / L2 / Linf
+    errors (%.10e), orders and error table equal the reference program's, character for
+    character (src/RP_FVM.c:974-1072, include/Error.h:129-329)."""
+    d = np.load(GOLDEN, allow_pickle=False)
+    cf = tmp_path / "ControlFile.txt"
+    cf.write_text(str(d[name + "__control"]).replace("@OUT@", str(tmp_path / "out")))
+    r = subprocess.run([build.DRIVER, str(cf)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ref = str(d[name + "__stdout"])
+    got = r.stdout[r.stdout.index("L1 Error = "):]
+    if got != ref:
+        gl, rl = got.splitlines(), ref.splitlines()
+        bad = [i for i, (a, b) in enumerate(zip(gl, rl)) if a != b] or [min(len(gl), len(rl))]
+        pytest.fail(f"{len(bad)} lines differ, first at {bad[0]}: "
+                    f"{gl[bad[0]] if bad[0] < len(gl) else None!r} vs "
+                    f"{rl[bad[0]] if bad[0] < len(rl) else None!r}")
