@@ -104,7 +104,6 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
     const int f = GC - 1 + blockIdx.x * kStrip + t;
     const bool face_ok = i < g.R - GC && f <= g.C - GC - 1;
     double gf[4] = {0.0, 0.0, 0.0, 0.0};
-    double y[4] = {0.0, 0.0, 0.0, 0.0};
     int pending = 0;
     if (face_ok) {
         const size_t c0 = (size_t)i * g.ld + f;
@@ -113,8 +112,6 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
         for (int n = 0; n < 6; n++)
 #pragma unroll
             for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2)];
-#pragma unroll
-        for (int v = 0; v < 4; v++) y[v] = st[3][v];
         if (op.compute(st, gf)) {
 #pragma unroll
             for (int v = 0; v < 4; v++) G[(size_t)v * g.plane + c0] = gf[v];
@@ -129,6 +126,11 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
     const int pending_up = __shfl_down(pending, 1, 64);
     const int c = f + 1;
     if (t < kStrip && face_ok && c <= g.C - GC - 1) {
+        // the cell's stage input, read again (a cache hit) rather than kept live in
+        // registers across the face arithmetic
+        double y[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) y[v] = Y[(size_t)v * g.plane + (size_t)i * g.ld + c];
         const bool ok = cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu);
         if (rp.cnt && !ok && !pending && !pending_up) {
             const unsigned k = atomicAdd(&rp.cnt[0], 1u);

@@ -19,6 +19,15 @@ constexpr int GC = 4;  // GhostCell, include/Golbal.h:49
 #else
 #define FLUX_BOUNDS __launch_bounds__(256)
 #endif
+// The same knob for the mixed form's fused y-flux + update kernel alone.
+#ifndef RPFVM_FUSED_MX_WAVES
+#define RPFVM_FUSED_MX_WAVES RPFVM_FLUX_WAVES
+#endif
+#if RPFVM_FUSED_MX_WAVES > 0
+#define FUSED_MX_BOUNDS __launch_bounds__(256, RPFVM_FUSED_MX_WAVES)
+#else
+#define FUSED_MX_BOUNDS __launch_bounds__(256)
+#endif
 
 // Device grid geometry: planes of R rows x ld doubles; plane v at base + v*plane.
 struct Geom {

@@ -46,27 +46,31 @@ __device__ __forceinline__ double sqrt64(double x) {
 // ---------------------------------------------------------------- FP32 WENO etc.
 __device__ __forceinline__ float weno5_f(float v1, float v2, float v3, float v4, float v5, bool z) {
     const float eps = 1.0e-15f;
-    const float t1 = v1 - 2.0f * v2 + v3, t2 = v1 - 4.0f * v2 + 3.0f * v3;
-    const float t3 = v2 - 2.0f * v3 + v4, t4 = v2 - v4;
-    const float t5 = v3 - 2.0f * v4 + v5, t6 = 3.0f * v3 - 4.0f * v4 + v5;
-    const float b1 = (13.0f / 12.0f) * t1 * t1 + 0.25f * t2 * t2;
-    const float b2 = (13.0f / 12.0f) * t3 * t3 + 0.25f * t4 * t4;
-    const float b3 = (13.0f / 12.0f) * t5 * t5 + 0.25f * t6 * t6;
+    // second differences and one-sided first differences of the three stencils
+    const float t1 = fmaf(-2.0f, v2, v1) + v3, t2 = fmaf(3.0f, v3, fmaf(-4.0f, v2, v1));
+    const float t3 = fmaf(-2.0f, v3, v2) + v4, t4 = v2 - v4;
+    const float t5 = fmaf(-2.0f, v4, v3) + v5, t6 = fmaf(3.0f, v3, fmaf(-4.0f, v4, v5));
+    const float b1 = fmaf((13.0f / 12.0f) * t1, t1, (0.25f * t2) * t2);
+    const float b2 = fmaf((13.0f / 12.0f) * t3, t3, (0.25f * t4) * t4);
+    const float b3 = fmaf((13.0f / 12.0f) * t5, t5, (0.25f * t6) * t6);
     float a1, a2, a3;
     if (z) {  // WENO5-Z, exponent 1
         const float tau = fabsf(b1 - b3);
-        a1 = 0.1f * (1.0f + tau * rcp(b1 + eps));
-        a2 = 0.6f * (1.0f + tau * rcp(b2 + eps));
-        a3 = 0.3f * (1.0f + tau * rcp(b3 + eps));
+        a1 = fmaf(0.1f * tau, rcp(b1 + eps), 0.1f);
+        a2 = fmaf(0.6f * tau, rcp(b2 + eps), 0.6f);
+        a3 = fmaf(0.3f * tau, rcp(b3 + eps), 0.3f);
     } else {  // WENO5-JS
         const float i1 = rcp(eps + b1), i2 = rcp(eps + b2), i3 = rcp(eps + b3);
-        a1 = 0.1f * i1 * i1;
-        a2 = 0.6f * i2 * i2;
-        a3 = 0.3f * i3 * i3;
+        a1 = (0.1f * i1) * i1;
+        a2 = (0.6f * i2) * i2;
+        a3 = (0.3f * i3) * i3;
     }
     const float is = rcp(a1 + a2 + a3) * (1.0f / 6.0f);
-    return (a1 * (2.0f * v1 - 7.0f * v2 + 11.0f * v3) + a2 * (-v2 + 5.0f * v3 + 2.0f * v4)
-            + a3 * (2.0f * v3 + 5.0f * v4 - v5)) * is;
+    // candidate stencils 2 v1 - 7 v2 + 11 v3, -v2 + 5 v3 + 2 v4, 2 v3 + 5 v4 - v5
+    const float p1 = fmaf(2.0f, v1, fmaf(-7.0f, v2, 11.0f * v3));
+    const float p2 = fmaf(5.0f, v3, fmaf(2.0f, v4, -v2));
+    const float p3 = fmaf(2.0f, v3, fmaf(5.0f, v4, -v5));
+    return fmaf(a1, p1, fmaf(a2, p2, a3 * p3)) * is;
 }
 
 __device__ __forceinline__ float weno3_f(float v1, float v2, float v3) {
@@ -126,49 +130,84 @@ __device__ __forceinline__ bool admissible64(double g, const double c[4]) {
     return c[0] > 0.0 && s.p > 0.0 && isfinite(s.p) && isfinite(s.u) && isfinite(s.v);
 }
 
-// Roe-averaged left/right eigenvectors in FP64 (the basis of Compute_Eigen_2DX/2DY,
-// include/Characteriz.h:346-392,448-494).  The Roe sound speed is formed without
-// cancellation, c~^2 = (sqrt(rho_a) c_a^2 + sqrt(rho_b) c_b^2)/(sqrt(rho_a)+sqrt(rho_b))
+// The Roe-averaged characteristic basis in FP64 (the eigenvectors of
+// Compute_Eigen_2DX/2DY, include/Characteriz.h:346-392,448-494), kept as the scalars
+// its rows are built from.  The Roe sound speed is formed without cancellation,
+// c~^2 = (sqrt(rho_a) c_a^2 + sqrt(rho_b) c_b^2)/(sqrt(rho_a)+sqrt(rho_b))
 //   + (gamma-1)/2 sqrt(rho_a rho_b)/(sqrt(rho_a)+sqrt(rho_b))^2 |u_b - u_a|^2,
-// which equals (gamma-1)(H~ - |u~|^2/2) for an ideal gas; then b2 = b1 |u~|^2/2,
-// H~ - |u~|^2 = c~^2/(gamma-1) - |u~|^2/2 and H~ b1 - 1 = b2.
+// which equals (gamma-1)(H~ - |u~|^2/2) for an ideal gas; then b1 = (gamma-1)/c~^2,
+// b2 = b1 |u~|^2/2, H~ = c~^2/(gamma-1) + |u~|^2/2.  With n the face-normal velocity
+// component and t the tangential one, the left eigenvectors (rows) are, for x faces,
+//   l0 = ((b2 + n/c)/2, -(b1 u + 1/c)/2, -b1 v/2, b1/2),  l1 = (H~ - |u~|^2, u, v, -1),
+//   l2 = (v, 0, -1, 0),  l3 = ((b2 - n/c)/2, (-b1 u + 1/c)/2, -b1 v/2, b1/2)
+// (y faces: u and v exchange roles in l0, l3 and l2 = (-u, 1, 0, 0)), and the right
+// eigenvectors (columns) r0 = (1, u - c, v, H~ - u c), r1 = b1 (1, u, v, |u~|^2/2),
+// r2 = (0, 0, -1, -v), r3 = (1, u + c, v, H~ + u c) for x faces (y faces:
+// r0 = (1, u, v - c, H~ - v c), r2 = (0, 1, 0, u), r3 = (1, u, v + c, H~ + v c)).
+struct Basis64 {
+    double u, v, c, nc;    // Roe velocity, sound speed, normal velocity x c
+    double b1, b2, H;      // right-eigenvector scalars
+    double hb1, hb2, hic;  // b1/2, b2/2, 1/(2c)
+    double Hmq2;           // H~ - |u~|^2
+};
+
 template <int DIR>
-__device__ __forceinline__ void face_eigen64(double g, const Prim64& A, const Prim64& B,
-                                             double EL[4][4], double ER[4][4]) {
+__device__ __forceinline__ Basis64 face_basis64(double g, const Prim64& A, const Prim64& B) {
     const double gm1 = g - 1.0;
     const double sa = sqrt64(A.r), sb = sqrt64(B.r);
     const double iss = rcp64(sa + sb);
-    const double u = (sa * A.u + sb * B.u) * iss, v = (sa * A.v + sb * B.v) * iss;
+    Basis64 e;
+    e.u = (sa * A.u + sb * B.u) * iss;
+    e.v = (sa * A.v + sb * B.v) * iss;
     const double du = B.u - A.u, dv = B.v - A.v;
     double c2 = (sa * g * A.p * A.ir + sb * g * B.p * B.ir) * iss
               + 0.5 * gm1 * sa * sb * iss * iss * (du * du + dv * dv);
     c2 = fmax(c2, 1.0e-300);
-    const double q2 = u * u + v * v;
+    const double q2 = e.u * e.u + e.v * e.v;
     const double ic2 = rcp64(c2);
-    const double c = sqrt64(c2), ic = c * ic2;
-    const double b1 = gm1 * ic2;
-    const double b2 = 0.5 * b1 * q2;
+    e.c = sqrt64(c2);
+    e.nc = (DIR == 1 ? e.u : e.v) * e.c;
+    e.b1 = gm1 * ic2;
+    e.b2 = 0.5 * e.b1 * q2;
     const double igm1 = 1.0 / gm1;
-    const double H = c2 * igm1 + 0.5 * q2;
-    const double Hmq2 = c2 * igm1 - 0.5 * q2;
+    e.H = c2 * igm1 + 0.5 * q2;
+    e.Hmq2 = c2 * igm1 - 0.5 * q2;
+    e.hb1 = 0.5 * e.b1;
+    e.hb2 = 0.5 * e.b2;
+    e.hic = 0.5 * (e.c * ic2);
+    return e;
+}
+
+// w = L d for one stencil cell, from the row structure: with P = u d1 + v d2 - d3,
+// w1 = (H~ - |u~|^2) d0 + P, A = (b2 d0 - b1 P)/2, B = (n d0 - d_n)/(2c),
+// w0 = A + B, w3 = A - B, w2 = v d0 - d2 (x) or d1 - u d0 (y).
+template <int DIR>
+__device__ __forceinline__ void project64(const Basis64& e, const double d[4], float w[4]) {
+    const double P = fma(e.u, d[1], fma(e.v, d[2], -d[3]));
+    const double A = fma(e.hb2, d[0], -e.hb1 * P);
+    const double B = DIR == 1 ? e.hic * fma(e.u, d[0], -d[1]) : e.hic * fma(e.v, d[0], -d[2]);
+    w[0] = (float)(A + B);
+    w[1] = (float)fma(e.Hmq2, d[0], P);
+    w[2] = (float)(DIR == 1 ? fma(e.v, d[0], -d[2]) : fma(-e.u, d[0], d[1]));
+    w[3] = (float)(A - B);
+}
+
+// dU = R w: with S = w0 + w3 and D = w3 - w0, dU0 = S + b1 w1 and
+// x: dU1 = u dU0 + c D, dU2 = v dU0 - w2, dU3 = H~ S + b2 w1 - v w2 + u c D;
+// y: dU1 = u dU0 + w2, dU2 = v dU0 + c D, dU3 = H~ S + b2 w1 + u w2 + v c D.
+template <int DIR>
+__device__ __forceinline__ void unproject64(const Basis64& e, const float w[4], double dU[4]) {
+    const double w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    const double S = w0 + w3, D = w3 - w0;
+    dU[0] = fma(e.b1, w1, S);
     if constexpr (DIR == 1) {
-        EL[0][0] = 0.5 * (b2 + u * ic); EL[0][1] = -0.5 * (b1 * u + ic); EL[0][2] = -0.5 * b1 * v; EL[0][3] = 0.5 * b1;
-        EL[1][0] = Hmq2;                EL[1][1] = u;                     EL[1][2] = v;                EL[1][3] = -1.0;
-        EL[2][0] = v;                   EL[2][1] = 0.0;                   EL[2][2] = -1.0;             EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - u * ic); EL[3][1] = 0.5 * (-b1 * u + ic);  EL[3][2] = -0.5 * b1 * v; EL[3][3] = 0.5 * b1;
-        ER[0][0] = 1.0;       ER[0][1] = b1;     ER[0][2] = 0.0;  ER[0][3] = 1.0;
-        ER[1][0] = u - c;     ER[1][1] = u * b1; ER[1][2] = 0.0;  ER[1][3] = u + c;
-        ER[2][0] = v;         ER[2][1] = v * b1; ER[2][2] = -1.0; ER[2][3] = v;
-        ER[3][0] = H - u * c; ER[3][1] = b2;     ER[3][2] = -v;   ER[3][3] = H + u * c;
+        dU[1] = fma(e.u, dU[0], e.c * D);
+        dU[2] = fma(e.v, dU[0], -w2);
+        dU[3] = fma(e.H, S, fma(e.b2, w1, fma(-e.v, w2, e.nc * D)));
     } else {
-        EL[0][0] = 0.5 * (b2 + v * ic); EL[0][1] = -0.5 * b1 * u; EL[0][2] = -0.5 * (b1 * v + ic); EL[0][3] = 0.5 * b1;
-        EL[1][0] = Hmq2;                EL[1][1] = u;             EL[1][2] = v;                      EL[1][3] = -1.0;
-        EL[2][0] = -u;                  EL[2][1] = 1.0;           EL[2][2] = 0.0;                    EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - v * ic); EL[3][1] = -0.5 * b1 * u; EL[3][2] = 0.5 * (-b1 * v + ic);  EL[3][3] = 0.5 * b1;
-        ER[0][0] = 1.0;       ER[0][1] = b1;     ER[0][2] = 0.0; ER[0][3] = 1.0;
-        ER[1][0] = u;         ER[1][1] = u * b1; ER[1][2] = 1.0; ER[1][3] = u;
-        ER[2][0] = v - c;     ER[2][1] = v * b1; ER[2][2] = 0.0; ER[2][3] = v + c;
-        ER[3][0] = H - v * c; ER[3][1] = b2;     ER[3][2] = u;   ER[3][3] = H + v * c;
+        dU[1] = fma(e.u, dU[0], w2);
+        dU[2] = fma(e.v, dU[0], e.c * D);
+        dU[3] = fma(e.H, S, fma(e.b2, w1, fma(e.u, w2, e.nc * D)));
     }
 }
 
@@ -187,28 +226,23 @@ __device__ __forceinline__ void reconstruct_face(double g, double delta, const d
         for (int v = 0; v < 4; v++) d[n][v] = st[n][v] - ref[v];
     const float df = (float)delta;
     if constexpr (CHAR) {
-        double EL[4][4], ER[4][4];
-        face_eigen64<DIR>(g, prim64(g, st[2]), prim64(g, st[3]), EL, ER);
+        const Basis64 e = face_basis64<DIR>(g, prim64(g, st[2]), prim64(g, st[3]));
+        float w[6][4];
+#pragma unroll
+        for (int n = 0; n < 6; n++) project64<DIR>(e, d[n], w[n]);
         float wl[4], wr[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            float w[6];
-#pragma unroll
-            for (int n = 0; n < 6; n++)
-                w[n] = (float)(d[n][0] * EL[k][0] + d[n][1] * EL[k][1] + d[n][2] * EL[k][2]
-                               + d[n][3] * EL[k][3]);
-            recon_scalar_f<ORDER, true>(w, df, wl[k], wr[k]);
+            const float s[6] = {w[0][k], w[1][k], w[2][k], w[3][k], w[4][k], w[5][k]};
+            recon_scalar_f<ORDER, true>(s, df, wl[k], wr[k]);
         }
+        double dl[4], dr[4];
+        unproject64<DIR>(e, wl, dl);
+        unproject64<DIR>(e, wr, dr);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            double l = 0.0, r = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                l += (double)wl[k] * ER[v][k];
-                r += (double)wr[k] * ER[v][k];
-            }
-            UL[v] = ref[v] + l;
-            UR[v] = ref[v] + r;
+            UL[v] = ref[v] + dl[v];
+            UR[v] = ref[v] + dr[v];
         }
     } else {
 #pragma unroll

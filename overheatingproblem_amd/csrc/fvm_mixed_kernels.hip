@@ -103,7 +103,7 @@ struct FaceOpMx {
 };
 
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ FLUX_BOUNDS void k_yflux_update_mixed(const double* Un, const double* Y,
+__global__ FUSED_MX_BOUNDS void k_yflux_update_mixed(const double* Un, const double* Y,
                                                             double* out,
                                                             const double* __restrict__ F,
                                                             double* G, Geom g, Phys ph, double dt,
