@@ -183,6 +183,8 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ph.gm1 = c.gamma - 1.0;
     x->ph.gr = c.gamma / (c.gamma - 1.0);
     x->ph.six = 6.0;
+    x->ph.idx = 1.0 / c.dx;
+    x->ph.idy = 1.0 / c.dy;
     x->ph.source = c.source ? 1 : 0;
     x->ph.positivity = c.positivity ? 1 : 0;
     x->ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;

@@ -195,6 +195,8 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         d.ph.gm1 = c.gamma - 1.0;
         d.ph.gr = c.gamma / (c.gamma - 1.0);
         d.ph.six = 6.0;
+        d.ph.idx = 1.0 / c.dx;
+        d.ph.idy = 1.0 / c.dy;
         d.ph.source = c.source ? 1 : 0;
         d.ph.positivity = 0;
         d.ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;

@@ -102,7 +102,7 @@ __device__ __forceinline__ void combine(const Phys& ph, double dt, int kind, con
         S[2] = rho * ph.gravity;
         S[3] = rho * ph.gravity * v;
     }
-    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+    const double idx = ph.idx, idy = ph.idy;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const double L = -(fu[k] - fl[k]) * idx - (gu[k] - gl[k]) * idy + S[k];

@@ -33,7 +33,7 @@ __device__ __forceinline__ bool cell_update(const double* Un, const double* Y, d
         S[2] = rho * ph.gravity;
         S[3] = rho * ph.gravity * v;
     }
-    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
+    const double idx = ph.idx, idy = ph.idy;
     double W[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -58,12 +58,12 @@ __device__ __forceinline__ bool cell_update(const double* Un, const double* Y, d
 
 // Residual + stage combine of cell (i, c) with the y-face fluxes given in registers
 // (gl = G_{c-1/2}, gu = G_{c+1/2}) and the cell's own state y[] (already loaded).
-// Returns admissibility of the forward-Euler state.
-__device__ __forceinline__ bool cell_update_regs(const double* Un, double* out,
+// Leaves the forward-Euler state Y + dt L in W (for the positivity safeguard's test).
+__device__ __forceinline__ void cell_update_regs(const double* Un, double* out,
                                                  const double* __restrict__ F, const Geom& g,
                                                  const Phys& ph, double dt, int kind, int i,
                                                  int c, const double y[4], const double gl[4],
-                                                 const double gu[4]) {
+                                                 const double gu[4], double W[4]) {
     const size_t o0 = (size_t)i * g.ld + c;
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
@@ -72,8 +72,7 @@ __device__ __forceinline__ bool cell_update_regs(const double* Un, double* out,
         S[2] = rho * ph.gravity;
         S[3] = rho * ph.gravity * v;
     }
-    const double idx = 1.0 / ph.dx, idy = 1.0 / ph.dy;
-    double W[4];
+    const double idx = ph.idx, idy = ph.idy;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const size_t o = (size_t)k * g.plane + o0;
@@ -88,13 +87,46 @@ __device__ __forceinline__ bool cell_update_regs(const double* Un, double* out,
         }
         out[o] = r;
     }
-    const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
-    return admissible(kc, W);
+}
+
+// cell_update_regs with the x fluxes (fl = F_{i-1/2}, fu = F_{i+1/2}) and U^n already
+// in registers (un is not read by UPD_EULER).
+__device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, const Phys& ph,
+                                                double dt, int kind, int i, int c,
+                                                const double y[4], const double un[4],
+                                                const double fl[4], const double fu[4],
+                                                const double gl[4], const double gu[4],
+                                                double W[4]) {
+    const size_t o0 = (size_t)i * g.ld + c;
+    double S[4] = {0.0, 0.0, 0.0, 0.0};
+    if (ph.source) {
+        const double rho = y[0];
+        const double v = y[2] / rho;
+        S[2] = rho * ph.gravity;
+        S[3] = rho * ph.gravity * v;
+    }
+    const double idx = ph.idx, idy = ph.idy;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double L = -(fu[k] - fl[k]) * idx - (gu[k] - gl[k]) * idy + S[k];
+        W[k] = y[k] + dt * L;
+        double r;
+        switch (kind) {
+            case UPD_EULER: r = y[k] + dt * L; break;
+            case UPD_RK2_2: r = 0.5 * un[k] + 0.5 * (y[k] + dt * L); break;
+            case UPD_RK3_2: r = (3.0 / 4.0) * un[k] + (1.0 / 4.0) * (y[k] + dt * L); break;
+            default: r = (1.0 / 3.0) * un[k] + (2.0 / 3.0) * (y[k] + dt * L); break;
+        }
+        out[(size_t)k * g.plane + o0] = r;
+    }
 }
 
 // FaceOp::compute(st, f) returns false when the face must be recomputed later (mixed
-// form); FaceOp::defer(c0) records such a face.
-template <typename FaceOp>
+// form); FaceOp::defer(c0) records such a face.  With PREFETCH the cell's x fluxes and
+// U^n are loaded before the face arithmetic, so that their latency overlaps it (24 more
+// VGPRs: it pays where the kernel keeps its occupancy, the mixed form at 4 waves/SIMD,
+// and loses where it does not, the FP64 form).
+template <bool PREFETCH, typename FaceOp>
 __device__ __forceinline__ void yflux_update_body(const double* Un, const double* Y, double* out,
                                                   const double* __restrict__ F, double* G,
                                                   const Geom& g, const Phys& ph, double dt,
@@ -103,6 +135,19 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
     const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
     const int f = GC - 1 + blockIdx.x * kStrip + t;
     const bool face_ok = i < g.R - GC && f <= g.C - GC - 1;
+    const int c = f + 1;
+    const bool cell_ok = t < kStrip && face_ok && c <= g.C - GC - 1;
+    double fl[4] = {0.0, 0.0, 0.0, 0.0}, fu[4] = {0.0, 0.0, 0.0, 0.0};
+    double un[4] = {0.0, 0.0, 0.0, 0.0};
+    if (PREFETCH && cell_ok) {
+        const size_t o0 = (size_t)i * g.ld + c;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fu[k] = F[(size_t)k * g.plane + o0];
+            fl[k] = F[(size_t)k * g.plane + o0 - g.ld];
+            if (kind != UPD_EULER) un[k] = Un[(size_t)k * g.plane + o0];
+        }
+    }
     double gf[4] = {0.0, 0.0, 0.0, 0.0};
     int pending = 0;
     if (face_ok) {
@@ -124,17 +169,21 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
 #pragma unroll
     for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
     const int pending_up = __shfl_down(pending, 1, 64);
-    const int c = f + 1;
-    if (t < kStrip && face_ok && c <= g.C - GC - 1) {
+    if (cell_ok) {
         // the cell's stage input, read again (a cache hit) rather than kept live in
         // registers across the face arithmetic
         double y[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) y[v] = Y[(size_t)v * g.plane + (size_t)i * g.ld + c];
-        const bool ok = cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu);
-        if (rp.cnt && !ok && !pending && !pending_up) {
-            const unsigned k = atomicAdd(&rp.cnt[0], 1u);
-            rp.list[0][k] = (i - GC) * rp.ny + (c - GC);
+        double W[4];
+        if constexpr (PREFETCH) cell_update_pre(out, g, ph, dt, kind, i, c, y, un, fl, fu, gf, gu, W);
+        else cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu, W);
+        if (rp.cnt && !pending && !pending_up) {
+            const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+            if (!admissible(kc, W)) {
+                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+                rp.list[0][k] = (i - GC) * rp.ny + (c - GC);
+            }
         }
     }
 }

@@ -164,7 +164,7 @@ __global__ FLUX_BOUNDS void k_yflux_update(const double* Un, const double* Y,
                                                       int kind, Repair rp,
                                                       unsigned* __restrict__ faults) {
     FaceOp64<ORDER, CHAR, SOLVER> op{ph, make_consts(ph), faults};
-    yflux_update_body(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
+    yflux_update_body<false>(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
 }
 
 // ------------------------------------------------------------------ update

@@ -121,7 +121,7 @@ __device__ __forceinline__ Prim64 prim64(double g, const double c[4]) {
     s.ir = rcp64(c[0]);
     s.u = c[1] * s.ir;
     s.v = c[2] * s.ir;
-    s.p = (c[3] - 0.5 * (c[1] * s.u + c[2] * s.v)) * (g - 1.0);
+    s.p = fma(-0.5, fma(c[1], s.u, c[2] * s.v), c[3]) * (g - 1.0);
     return s;
 }
 
@@ -157,21 +157,22 @@ __device__ __forceinline__ Basis64 face_basis64(double g, const Prim64& A, const
     const double sa = sqrt64(A.r), sb = sqrt64(B.r);
     const double iss = rcp64(sa + sb);
     Basis64 e;
-    e.u = (sa * A.u + sb * B.u) * iss;
-    e.v = (sa * A.v + sb * B.v) * iss;
+    e.u = fma(sa, A.u, sb * B.u) * iss;
+    e.v = fma(sa, A.v, sb * B.v) * iss;
     const double du = B.u - A.u, dv = B.v - A.v;
-    double c2 = (sa * g * A.p * A.ir + sb * g * B.p * B.ir) * iss
-              + 0.5 * gm1 * sa * sb * iss * iss * (du * du + dv * dv);
+    const double a2A = g * (A.p * A.ir), a2B = g * (B.p * B.ir);
+    const double siss = sa * iss, tiss = sb * iss;
+    double c2 = fma(siss, a2A, tiss * a2B) + (0.5 * gm1) * (siss * tiss) * fma(du, du, dv * dv);
     c2 = fmax(c2, 1.0e-300);
-    const double q2 = e.u * e.u + e.v * e.v;
+    const double q2 = fma(e.u, e.u, e.v * e.v);
     const double ic2 = rcp64(c2);
     e.c = sqrt64(c2);
     e.nc = (DIR == 1 ? e.u : e.v) * e.c;
     e.b1 = gm1 * ic2;
     e.b2 = 0.5 * e.b1 * q2;
     const double igm1 = 1.0 / gm1;
-    e.H = c2 * igm1 + 0.5 * q2;
-    e.Hmq2 = c2 * igm1 - 0.5 * q2;
+    e.H = fma(c2, igm1, 0.5 * q2);
+    e.Hmq2 = fma(c2, igm1, -0.5 * q2);
     e.hb1 = 0.5 * e.b1;
     e.hb2 = 0.5 * e.b2;
     e.hic = 0.5 * (e.c * ic2);
@@ -287,7 +288,7 @@ __device__ __forceinline__ Speeds<DIR> einfeldt(double g, const Prim64& L, const
     const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
     const float dq = (float)(qR - qL);
     const float eta = 0.5f * sqL * sqR * iss * iss;
-    const float d = fsqrt(fmaxf((sqL * aL2 + sqR * aR2) * iss + eta * dq * dq, 0.0f));
+    const float d = fsqrt(fmaxf(fmaf(fmaf(sqL, aL2, sqR * aR2), iss, eta * dq * dq), 0.0f));
     const float nL = sqR * dq * iss, nR = -sqL * dq * iss;  // q~ - q_L, q~ - q_R
     Speeds<DIR> s;
     s.aL = nL - d;

@@ -109,7 +109,7 @@ __global__ FUSED_MX_BOUNDS void k_yflux_update_mixed(const double* Un, const dou
                                                             double* G, Geom g, Phys ph, double dt,
                                                             int kind, Repair rp, FaceFix fix) {
     FaceOpMx<ORDER, CHAR, SOLVER> op{ph, fix};
-    yflux_update_body(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
+    yflux_update_body<true>(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
 }
 
 // Cells on either side of each recomputed y face: residual + stage combine again, with
