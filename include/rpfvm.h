@@ -73,8 +73,10 @@ enum {
     RPFVM_FAULT_NONFINITE_DT = 2,    /* dt reduction produced a non-finite or non-positive dt */
     RPFVM_FAULT_MIXED_FALLBACK = 4,  /* mixed form recomputed at least one face in FP64
                                         (informational, not an error) */
-    RPFVM_FAULT_POSITIVITY = 8       /* the positivity safeguard left a cell with rho <= 0
+    RPFVM_FAULT_POSITIVITY = 8,      /* the positivity safeguard left a cell with rho <= 0
                                         or p <= 0 */
+    RPFVM_FAULT_GRID_SYNC = 16       /* internal: a grid barrier of the safeguard's
+                                        cooperative kernel did not complete */
 };
 
 /* rpfvm_config.debug_flags bits. */
