@@ -16,6 +16,24 @@ namespace fvm {
 
 constexpr int kStrip = 63;  // cells updated per 64-lane row
 
+// Block (x, y) of a 2-D grid, remapped so that each XCD works on one contiguous range of
+// the grid.  Workgroups are dispatched to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2: with the identity map, blocks that share stencil rows and cache
+// lines (the neighbours in x and y) land on different XCDs and fetch the shared lines
+// from HBM once each.  Here linear id L runs as logical block (L % 8) * (N / 8) + L / 8,
+// so neighbouring logical blocks run on the same XCD at nearly the same time (a tail of
+// N % 8 blocks keeps the identity map).
+struct Tile2 {
+    int x, y;
+};
+__device__ __forceinline__ Tile2 xcd_tile() {
+    const unsigned nx = gridDim.x, n = gridDim.x * gridDim.y;
+    const unsigned lin = blockIdx.y * nx + blockIdx.x;
+    const unsigned per = n / 8;
+    const unsigned l = lin < per * 8 ? (lin % 8) * per + lin / 8 : lin;
+    return Tile2{(int)(l % nx), (int)(l / nx)};
+}
+
 // Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
 // fused with the RK stage combine (include/time_advance.h:697, 757-777, 832-872) for
 // cell (i, j).  `out` may alias `Un` (the last stage writes the state in place): each
@@ -132,8 +150,9 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
                                                   const Geom& g, const Phys& ph, double dt,
                                                   int kind, const Repair& rp, FaceOp& op) {
     const int t = threadIdx.x;
-    const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
-    const int f = GC - 1 + blockIdx.x * kStrip + t;
+    const Tile2 b = xcd_tile();
+    const int i = GC + b.y * blockDim.y + threadIdx.y;
+    const int f = GC - 1 + b.x * kStrip + t;
     const bool face_ok = i < g.R - GC && f <= g.C - GC - 1;
     const int c = f + 1;
     const bool cell_ok = t < kStrip && face_ok && c <= g.C - GC - 1;
