@@ -207,9 +207,12 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
     }
 }
 
+constexpr int kFusedRows = RPFVM_FUSED_ROWS;  // grid rows per workgroup of the fused kernels
+
 inline dim3 yflux_update_grid(const Geom& g) {
     const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
-    return dim3((nj + kStrip - 1) / kStrip, (ni + 3) / 4);
+    return dim3((nj + kStrip - 1) / kStrip, (ni + kFusedRows - 1) / kFusedRows);
 }
+inline dim3 yflux_update_block() { return dim3(64, kFusedRows); }
 
 }  // namespace fvm

@@ -158,7 +158,7 @@ struct FaceOp64 {
 };
 
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ FLUX_BOUNDS void k_yflux_update(const double* Un, const double* Y,
+__global__ FUSED_BOUNDS void k_yflux_update(const double* Un, const double* Y,
                                                       double* out, const double* __restrict__ F,
                                                       double* G, Geom g, Phys ph, double dt,
                                                       int kind, Repair rp,
@@ -470,7 +470,7 @@ static void launch_yflux_update(const double* Un, const double* Y, double* out, 
                                 double* G, const Geom& g, const Phys& ph, double dt, int kind,
                                 const Repair& rp, const FaceFix&, unsigned, unsigned* faults,
                                 hipStream_t s) {
-    hipLaunchKernelGGL((k_yflux_update<ORDER, CHAR, SOLVER>), yflux_update_grid(g), dim3(64, 4), 0,
+    hipLaunchKernelGGL((k_yflux_update<ORDER, CHAR, SOLVER>), yflux_update_grid(g), yflux_update_block(), 0,
                        s, Un, Y, out, F, G, g, ph, dt, kind, rp, faults);
 }
 

@@ -19,14 +19,23 @@ constexpr int GC = 4;  // GhostCell, include/Golbal.h:49
 #else
 #define FLUX_BOUNDS __launch_bounds__(256)
 #endif
-// The same knob for the mixed form's fused y-flux + update kernel alone.
+// Grid rows per workgroup of the fused y-flux + update kernels (64 lanes per row).
+#ifndef RPFVM_FUSED_ROWS
+#define RPFVM_FUSED_ROWS 2
+#endif
+#if RPFVM_FLUX_WAVES > 0
+#define FUSED_BOUNDS __launch_bounds__(64 * RPFVM_FUSED_ROWS, RPFVM_FLUX_WAVES)
+#else
+#define FUSED_BOUNDS __launch_bounds__(64 * RPFVM_FUSED_ROWS)
+#endif
+// The waves knob for the mixed form's fused kernel alone.
 #ifndef RPFVM_FUSED_MX_WAVES
 #define RPFVM_FUSED_MX_WAVES RPFVM_FLUX_WAVES
 #endif
 #if RPFVM_FUSED_MX_WAVES > 0
-#define FUSED_MX_BOUNDS __launch_bounds__(256, RPFVM_FUSED_MX_WAVES)
+#define FUSED_MX_BOUNDS __launch_bounds__(64 * RPFVM_FUSED_ROWS, RPFVM_FUSED_MX_WAVES)
 #else
-#define FUSED_MX_BOUNDS __launch_bounds__(256)
+#define FUSED_MX_BOUNDS __launch_bounds__(64 * RPFVM_FUSED_ROWS)
 #endif
 
 // Device grid geometry: planes of R rows x ld doubles; plane v at base + v*plane.

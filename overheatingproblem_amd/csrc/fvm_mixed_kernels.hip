@@ -150,7 +150,7 @@ static void launch_yflux_update_mixed(const double* Un, const double* Y, double*
                                       double dt, int kind, const Repair& rp, const FaceFix& fix,
                                       unsigned stamp, unsigned* faults, hipStream_t s) {
     hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER>), yflux_update_grid(g),
-                       dim3(64, 4), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix);
+                       yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix);
     hipLaunchKernelGGL((k_flux_fix<2, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, G, g,
                        ph, fix, faults);
     launch_yfix_cells(Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, s);
