@@ -4,6 +4,7 @@
     python -m overheatingproblem_amd.profiling traffic <fetch_dir> <write_dir> \
         <cal_fetch_dir> <cal_write_dir> <cal_bytes> <name> "<commands>"
     python -m overheatingproblem_amd.profiling valu    <pmc_dir> <name> "<command>"
+    python -m overheatingproblem_amd.profiling accuracy <noh_full.json> <name> "<command>"
 
 `trace` turns a `rocprofv3 --kernel-trace --stats --output-format csv` directory into a
 markdown table of per-kernel calls and average durations.  `traffic` turns two
@@ -12,7 +13,8 @@ scaled by what it reads for a plain copy kernel of `cal_bytes` read and written 
 dispatch (overheatingproblem_amd/lib/hbm_probe), and merges the figures into
 profiles/hbm_traffic.json, which bench.py reports as roofline.traffic.  `valu` records
 per-dispatch SQ instruction counters (SQ_INSTS_VALU ...) in the same file, from which
-bench.py reports the VALU issue rate of the dominant kernel.
+bench.py reports the VALU issue rate of the dominant kernel.  `accuracy` writes the
+full-length Noh accuracy record from the JSON line of `bench.py --noh-full`.
 """
 from __future__ import annotations
 
@@ -127,6 +129,39 @@ def valu_markdown(v: dict, name: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def accuracy_markdown(d: dict, command: str) -> str:
+    """The Noh full-length accuracy record from `bench.py --noh-full`'s JSON line."""
+    runs = (("FP64", "fp64"), ("mixed (own dts)", "mixed"), ("mixed (FP64 dts)", "mixed_replay"),
+            ("FP64, rho perturbed 1e-12", "fp64_perturbed"))
+    n = d["workload"].split("-")[1]
+    lines = [f"# Noh {n} to t = {d['t_end']}: accuracy of the mixed form", "",
+             "Command (from the repository root):", "", "```", command, "```", "",
+             "Noh implosion, gamma 5/3, p0 = 1e-6, WENO5-char + HLLC, SSP-RK3, CFL 0.4, "
+             "positivity safeguard on, one MI355X.  Exact solution: rho = 16 behind the shock "
+             "(r < t/3); the wall-heating deficit is 1 - rho/16 near the origin.", "",
+             "| run | steps | wall s | rho(corner cell) | deficit corner | deficit r<0.05 | "
+             "rho plateau 0.1<r<0.15 |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for label, k in runs:
+        r = d[k]
+        lines.append(f"| {label} | {r['steps']} | {r['wall_s']:.2f} | {r['rho_corner']:.4f} | "
+                     f"{r['deficit_corner']:+.4f} | {r['deficit_r_lt_0.05']:+.5f} | "
+                     f"{r['rho_plateau_0.1_0.15']:.4f} |")
+    lines += ["", "Differences from the FP64 run (interior, absolute and relative to mean |FP64|):",
+              "", "| compared with FP64 | L1(rho) | rel | Linf(rho) | L1(p) | rel | Linf(p) |",
+              "|---|---:|---:|---:|---:|---:|---:|"]
+    for label, k in runs[1:]:
+        c = d[k + "_vs_fp64"]
+        lines.append(f"| {label} | {c['l1_rho']:.3e} | {c['l1_rho_rel']:.3e} | {c['linf_rho']:.3f} "
+                     f"| {c['l1_p']:.3e} | {c['l1_p_rel']:.3e} | {c['linf_p']:.3f} |")
+    ref = d["fp64_perturbed_vs_fp64"]["l1_rho"]
+    ratios = ", ".join(f"{label} {d[k + '_vs_fp64']['l1_rho'] / ref:.2f}" for label, k in runs[1:3])
+    lines += ["", "Reading: the Noh solution amplifies round-off at the origin (wall heating) and at "
+              "the shock, so over the full run FP64 itself, started from a density perturbed by "
+              "1e-12, ends this far from the unperturbed FP64 run; the mixed form's L1(rho) "
+              f"distance from FP64, in units of that FP64 self-distance: {ratios}."]
+    return "\n".join(lines) + "\n"
+
+
 def _merge(name: str, key: str, value) -> None:
     try:
         with open(TRAFFIC_JSON) as fh:
@@ -163,6 +198,15 @@ def main(argv):
             fh.write(traffic_markdown(t, name, commands))
         _merge(name, "calibration", t["calibration"])
         _merge(name, "kernels", t["kernels"])
+        print(path)
+        return 0
+    if len(argv) >= 4 and argv[0] == "accuracy":
+        _, jpath, name, command = argv[:4]
+        with open(jpath) as fh:
+            d = json.loads(fh.read().strip().splitlines()[-1])
+        path = os.path.join(PROFILES, f"{name}_accuracy.md")
+        with open(path, "w") as fh:
+            fh.write(accuracy_markdown(d, command))
         print(path)
         return 0
     if len(argv) >= 4 and argv[0] == "valu":
