@@ -223,18 +223,22 @@ int rpfvm_kernel_times(rpfvm_ctx *ctx, double ms[RPFVM_KCLASS_COUNT], int counts
  * stage of all active problems is one persistent launch driven by a work queue
  * of strips; the whole step is captured once in a HIP graph and replayed.  Each
  * problem's interior evolves bit-identically to an rpfvm_ctx with the same
- * configuration driven by rpfvm_advance_to(t_end); ghost layers are not stored
- * (boundary conditions are applied on load), so rpfvm_batch_get_state returns the
- * interior and leaves the ghost entries of U as rpfvm_batch_set_state wrote them.
+ * configuration driven by rpfvm_advance_to(t_end).  rpfvm_batch_get_state returns the
+ * interior and leaves the ghost entries of U as rpfvm_batch_set_state wrote them (the
+ * one-pass schedules apply the boundary conditions on load; the two-sweep schedule
+ * fills device ghost frames that are not copied back).
  * ------------------------------------------------------------------------- */
 typedef struct rpfvm_batch rpfvm_batch;
 
-enum { RPFVM_STAGING_DIRECT = 0, RPFVM_STAGING_LDS = 1 };
+/* Schedule of a stage: one marching pass (stencils from cache, or through an LDS row
+ * ring), or two sweeps (ghost fill, x fluxes to a buffer, y fluxes fused with the
+ * update). */
+enum { RPFVM_STAGING_DIRECT = 0, RPFVM_STAGING_LDS = 1, RPFVM_STAGING_SWEEPS = 2 };
 
 /* Launch shape of the stage kernel.  rpfvm_batch_tune measures candidates on the
  * batch's current state and keeps the fastest. */
 typedef struct rpfvm_batch_tuning {
-    int staging;        /* RPFVM_STAGING_*: stencils from cache or from an LDS row ring */
+    int staging;        /* RPFVM_STAGING_* */
     int rows;           /* rows a wavefront marches per strip (4..1024) */
     int waves_per_cu;   /* resident wavefronts of the persistent grid per CU (1..32) */
     int use_graph;      /* 1: replay the captured step graph; 0: launch eagerly */

@@ -39,7 +39,7 @@ EXPORTS = (
     "rpfvm_batch_tune", "rpfvm_batch_set_tuning", "rpfvm_batch_get_tuning", "rpfvm_batch_stream",
 )
 
-STAGING_DIRECT, STAGING_LDS = 0, 1
+STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS = 0, 1, 2
 
 
 class BatchTuning(ctypes.Structure):

@@ -23,6 +23,15 @@
 // integers), and k_batch_control turns it into each problem's dt, clock, activity and
 // the schedule of active strips before stage 1.  The step is a fixed sequence of
 // launches on one stream, captured once into a hipGraph and replayed.
+//
+// The two-sweep schedule (STAGING_SWEEPS) trades the fluxes-in-registers property for
+// the single-problem kernels' register profile: per stage, the ghost frames of the
+// stage input are filled (the four sides in order, as the single path), a persistent
+// sweep computes the x faces of every active strip into a flux buffer, and a second
+// persistent sweep computes the y faces and updates the cells with the x fluxes read
+// back (the marching kernel keeps both sweeps' state live and runs at 2 waves/SIMD).
+// Both schedules use the same strips, work counters and face arithmetic, so their
+// results are identical.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,7 +44,10 @@ namespace fvm {
 
 constexpr int kStripCols = 63;  // cells per strip row (64 y faces)
 
-enum : int { STAGING_DIRECT = 0, STAGING_LDS = 1 };
+// How a stage gets its stencils: the one-pass marching kernel reading the stage input
+// directly or through an LDS ring, or the two-sweep schedule (ghost fill, x-flux sweep to
+// a flux buffer, fused y-flux + update sweep).
+enum : int { STAGING_DIRECT = 0, STAGING_LDS = 1, STAGING_SWEEPS = 2 };
 
 // One problem of the batch (device-resident, read with scalar loads).
 struct BProb {
@@ -48,6 +60,7 @@ struct BProb {
     size_t off;           // offset in doubles of the problem's planes in each pool buffer
     int strips_y;         // ceil(ny / 63)
     int pad;
+    BcSide side[4];       // ghost fill of the two-sweep schedule (as the single path's)
 };
 
 // Per-problem state of the time loop.
@@ -65,6 +78,8 @@ struct BStat {
 struct BSched {
     unsigned queue[4];    // work counters of stages 1..3 (+1 spare)
     unsigned defer[4];    // deferred-cell counts of stages 1..3 (mixed form)
+    unsigned xqueue[4];   // two-sweep schedule: x-sweep work counters
+    unsigned xdefer[4];   // two-sweep schedule: deferred x faces (mixed form)
     int nact;             // active problems
     int total;            // active strips
     int active_left;      // problems with t < t_end after this step's dt
@@ -79,10 +94,13 @@ struct BatchArgs {
     int* act_prob;             // [nact] problem index of each active problem
     int* act_start;            // [nact + 1] first strip of each active problem
     double* pool[3];           // U, U1, U2 of every problem (problem k at + probs[k].off)
+    double* flux;              // x-face fluxes of the two-sweep schedule (same layout)
     unsigned long long* defer_list;  // (problem << 40) | (i << 20) | j of deferred cells
+    unsigned long long* xdefer_list; // (problem << 40) | (i << 20) | j of deferred x faces
     unsigned* faults;          // batch-wide OR of fault bits
     int nprob;
     int rows;                  // TX: rows per strip
+    int max_side;              // largest R or C of any problem (ghost-fill launch shape)
 };
 
 struct BatchStageLaunch {
@@ -102,6 +120,9 @@ BatchKernels select_batch(int order, bool characteristic, bool mixed);
 void launch_batch_control(const BatchArgs& a, int mode, hipStream_t s);
 void launch_batch_copy_back(const BatchArgs& a, hipStream_t s);
 void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s);
+// Two-sweep schedule: wave-speed maxima of every active problem's state (after RK1's
+// copy back; the other orders launch it from their last stage).
+void launch_batch_speeds_all(const BatchArgs& a, hipStream_t s);
 void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s);
 
 }  // namespace fvm

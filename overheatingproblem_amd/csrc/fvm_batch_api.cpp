@@ -1,10 +1,12 @@
 // fvm_batch_api.cpp -- the batching form's C ABI (include/rpfvm.h, rpfvm_batch_*).
 //
 // Device memory of a batch, allocated once:
-//   three state pools (U, U1, U2): every problem's four planes of R rows x ld doubles,
-//     problem k at offset probs[k].off (interior column 128-byte aligned)
+//   three state pools (U, U1, U2) and an x-flux pool (two-sweep schedule): every
+//     problem's four planes of R rows x ld doubles, problem k at offset probs[k].off
+//     (interior column 128-byte aligned)
 //   probs[count] (BProb), stat[count] (BStat), one BSched, the active-problem tables,
-//   the deferred-cell list of the mixed form (one entry per interior cell at most)
+//   the deferred-cell and deferred-x-face lists of the mixed form (one entry per interior
+//   cell / x face at most)
 // A step is: k_batch_control, then per RK stage one persistent k_batch_stage launch
 // (+ k_batch_fix in the mixed form), + k_batch_copy_back for RK1.  It contains no
 // memset and no host synchronisation, so it is captured once into a hipGraph
@@ -63,7 +65,10 @@ void enqueue_step(rpfvm_batch* b) {
     const BatchStageLaunch l = launch_shape(b);
     for (int s = 0; s < stages_of(b->time_order); s++)
         b->kern.stage(b->args, b->time_order, s, l, b->stream);
-    if (b->time_order == 1) launch_batch_copy_back(b->args, b->stream);
+    if (b->time_order == 1) {
+        launch_batch_copy_back(b->args, b->stream);
+        if (l.staging == STAGING_SWEEPS) launch_batch_speeds_all(b->args, b->stream);
+    }
 }
 
 void drop_graph(rpfvm_batch* b) {
@@ -109,8 +114,9 @@ int run_step(rpfvm_batch* b) {
 }
 
 int check_tuning(const rpfvm_batch_tuning* t) {
-    if (t->staging != RPFVM_STAGING_DIRECT && t->staging != RPFVM_STAGING_LDS)
-        return fail(RPFVM_ERR_INVALID, "staging must be 0 or 1");
+    if (t->staging != RPFVM_STAGING_DIRECT && t->staging != RPFVM_STAGING_LDS &&
+        t->staging != RPFVM_STAGING_SWEEPS)
+        return fail(RPFVM_ERR_INVALID, "staging must be 0, 1 or 2");
     if (t->rows < 4 || t->rows > 1024) return fail(RPFVM_ERR_INVALID, "rows must be in 4..1024");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32)
         return fail(RPFVM_ERR_INVALID, "waves_per_cu must be in 1..32");
@@ -209,6 +215,18 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
                 d.bcv[s][v] = c.bc[s] == RPFVM_BC_INFLOW ? c.inflow_state[v] : c.fixed_state[s][v];
         }
         d.strips_y = (c.ny + kStripCols - 1) / kStripCols;
+        for (int s = 0; s < 4; s++) {  // the single path's sides (fvm_api.cpp, rpfvm_create)
+            BcSide& bs = d.side[s];
+            bs.side = s;
+            bs.type = c.bc[s];
+            bs.lo = GC;
+            const int last = s < 2 ? d.g.C - GC : d.g.R - GC;
+            bs.hi = bs.type == RPFVM_BC_OUTFLOW ? last : last - 1;
+            for (int v = 0; v < 4; v++) bs.value[v] = d.bcv[s][v];
+            bs.nx = (double)c.nx;
+            bs.lx = c.lx > 0.0 ? c.lx : c.nx * c.dx;
+        }
+        b->args.max_side = std::max(b->args.max_side, std::max(d.g.R, d.g.C));
         off = (off + 31) / 32 * 32;  // 256-byte aligned region
         d.off = off + fvm::api::kFrontPad;
         off = d.off + 4 * d.g.plane + 32;
@@ -221,8 +239,12 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     const size_t sched_bytes = a256(sizeof(BSched));
     const size_t act_bytes = a256(sizeof(int) * (2 * (size_t)count + 1));
     const size_t defer_bytes = b->mixed ? a256(sizeof(unsigned long long) * cells) : 0;
-    const size_t total = 3 * pool_doubles * sizeof(double) + probs_bytes + stat_bytes +
-                         sched_bytes + act_bytes + 256 + defer_bytes;
+    size_t xfaces = 0;  // x faces of every problem: (nx + 1) ny
+    for (int k = 0; k < count; k++) xfaces += (size_t)(cfgs[k].nx + 1) * cfgs[k].ny;
+    const size_t xdefer_bytes = b->mixed ? a256(sizeof(unsigned long long) * xfaces) : 0;
+    // pools: U, U1, U2 and the two-sweep schedule's x-flux buffer
+    const size_t total = 4 * pool_doubles * sizeof(double) + probs_bytes + stat_bytes +
+                         sched_bytes + act_bytes + 256 + defer_bytes + xdefer_bytes;
 
     hipError_t he = hipGetDevice(&b->device);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
@@ -237,7 +259,8 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(he)));
     char* p = static_cast<char*>(b->pool);
     for (int q = 0; q < 3; q++) b->args.pool[q] = reinterpret_cast<double*>(p) + q * pool_doubles;
-    char* r = p + 3 * pool_doubles * sizeof(double);
+    b->args.flux = reinterpret_cast<double*>(p) + 3 * pool_doubles;
+    char* r = p + 4 * pool_doubles * sizeof(double);
     b->args.probs = reinterpret_cast<const BProb*>(r);
     BProb* dprobs = reinterpret_cast<BProb*>(r);
     r += probs_bytes;
@@ -251,6 +274,8 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     b->args.faults = reinterpret_cast<unsigned*>(r);
     r += 256;
     b->args.defer_list = b->mixed ? reinterpret_cast<unsigned long long*>(r) : nullptr;
+    r += defer_bytes;
+    b->args.xdefer_list = b->mixed ? reinterpret_cast<unsigned long long*>(r) : nullptr;
     b->args.nprob = count;
     b->args.rows = b->tuning.rows;
 
@@ -372,7 +397,7 @@ int rpfvm_batch_tune(rpfvm_batch* b, rpfvm_batch_tuning* chosen) {
     if (!b) return fail(RPFVM_ERR_INVALID, "null batch");
     const int rows_c[] = {8, 16, 32, 64, 128};
     const int waves_c[] = {4, 8, 12, 16};
-    const int stagings[] = {RPFVM_STAGING_DIRECT, RPFVM_STAGING_LDS};
+    const int stagings[] = {RPFVM_STAGING_DIRECT, RPFVM_STAGING_LDS, RPFVM_STAGING_SWEEPS};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));

@@ -9,9 +9,15 @@
 //   k_batch_fix       mixed form: the cells next to faces whose FP32 states were not
 //                     admissible, with those faces recomputed in FP64
 //   k_batch_copy_back RK1 only: U1 -> U over the active interiors
+//   two-sweep schedule (per stage):
+//   k_batch_ghosts    ghost frame of the stage input, one side of every active problem
+//   k_batch_xsweep    persistent: x-face fluxes of every active strip into a.flux
+//   k_batch_xfix      mixed form: x faces whose FP32 states were not admissible, in FP64
+//   k_batch_ysweep    persistent: y faces fused with the residual and stage combine
 #include <hip/hip_runtime.h>
 
 #include "fvm_batch.hpp"
+#include "fvm_bc.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 #include "fvm_mixed.hpp"
@@ -223,6 +229,8 @@ __global__ void k_batch_control(BatchArgs a, int mode) {
         for (int q = 0; q < 4; q++) {
             sc.queue[q] = 0u;
             sc.defer[q] = 0u;
+            sc.xqueue[q] = 0u;
+            sc.xdefer[q] = 0u;
         }
     }
 }
@@ -424,6 +432,239 @@ __global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int 
     }
 }
 
+// -------------------------------------------------------- two-sweep schedule
+// Ghost frame of side `side` of every active problem's stage input (pool[yi]), as the
+// single path's k_bc_side: grid (tangential blocks, 4 variables, problems).
+__global__ __launch_bounds__(64) void k_batch_ghosts(BatchArgs a, int side, int yi) {
+    for (int k = blockIdx.z; k < a.nprob; k += gridDim.z) {
+        if (!a.stat[k].active) continue;
+        const BProb& d = a.probs[k];
+        const BcSide s = d.side[side];
+        const int t = s.lo + blockIdx.x * blockDim.x + threadIdx.x;
+        if (t > s.hi) continue;
+        bc_side_point(a.pool[yi] + d.off, d.g, s, t, blockIdx.y);
+    }
+}
+
+// The x faces F[r][c] (between rows r and r + 1) of every active strip: lane t owns
+// column c = j0 + t (t < 63) and marches the strip's rows, as one row of k_flux<1> per
+// step (the first strip row of a problem also computes its lower boundary face).
+// Mixed form: a face whose FP32 states are not admissible goes to xdefer_list.
+template <int ORDER, bool CHAR, bool MIXED>
+__global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int yi) {
+    const int lane = threadIdx.x;
+    const int nact = a.sched->nact, total = a.sched->total;
+    unsigned* queue = &a.sched->xqueue[stage];
+    for (int trip = 0; trip <= total; trip++) {
+        const unsigned ticket = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const int item = uniform((int)ticket);
+        if (item >= total) break;
+        const int ai = uniform(owner(a, nact, item));
+        const int k = uniform(a.act_prob[ai]);
+        const int local = item - a.act_start[ai];
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const int solver = d.solver;
+        const double* Y = a.pool[yi] + d.off;
+        double* F = a.flux + d.off;
+        const int sx = local / d.strips_y, sy = local - sx * d.strips_y;
+        const int i0 = GC + sx * a.rows;
+        const int i1 = min(i0 + a.rows, g.R - GC);
+        const int c = GC + sy * kStripCols + lane;
+        const bool col_ok = lane < kStripCols && c <= g.C - GC - 1;
+        int fault = 0;
+        auto row_phys = [&]() {
+            const BProb* dq = a.probs + k;
+            asm volatile("" : "+s"(dq));
+            return dq->ph;
+        };
+        for (int r = i0 == GC ? GC - 1 : i0; r < i1; r++) {
+            if (!col_ok) continue;
+            // strides through an opaque copy, so that the 24 stencil offsets are formed
+            // at their loads instead of hoisted out of the row loop into live registers
+            size_t ld = g.ld, pl = g.plane;
+            asm volatile("" : "+v"(ld), "+v"(pl));
+            const size_t c0 = (size_t)opaque_row(r) * ld + c;
+            double st[6][4], f[4];
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * pl + c0 + (size_t)(n - 2) * ld];
+            const Phys pr = row_phys();
+            if constexpr (MIXED) {
+                if (!face_mx<1, ORDER, CHAR>(pr, solver, st, f)) {
+                    const unsigned q = atomicAdd(&a.sched->xdefer[stage], 1u);
+                    a.xdefer_list[q] = ((unsigned long long)k << 40) |
+                                       ((unsigned long long)r << 20) | (unsigned long long)c;
+                    continue;
+                }
+            } else {
+                fault |= face64<1, ORDER, CHAR>(make_consts(pr), pr, solver, st, f);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) F[(size_t)v * pl + c0] = f[v];
+        }
+        if (!MIXED) {
+            const unsigned long long any = __ballot(fault != 0);
+            if (any && lane == 0) {
+                atomicOr(a.faults, 1u);
+                atomicOr(&a.stat[k].faults, 1u);
+            }
+        }
+    }
+}
+
+// The deferred x faces recomputed with the FP64 path (k_flux_fix's arithmetic).
+template <int ORDER, bool CHAR>
+__global__ __launch_bounds__(256) void k_batch_xfix(BatchArgs a, int stage, int yi) {
+    const unsigned n = a.sched->xdefer[stage];
+    if (n && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.faults, 4u);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const unsigned long long e = a.xdefer_list[q];
+        const int k = (int)(e >> 40), r = (int)((e >> 20) & 0xFFFFF), c = (int)(e & 0xFFFFF);
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const double* Y = a.pool[yi] + d.off;
+        double* F = a.flux + d.off;
+        const size_t c0 = (size_t)r * g.ld + c;
+        double st[6][4], f[4];
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (size_t)(m - 2) * g.ld];
+        if (face64<1, ORDER, CHAR>(make_consts(d.ph), d.ph, d.solver, st, f)) {
+            atomicOr(a.faults, 1u);
+            atomicOr(&a.stat[k].faults, 1u);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+    }
+}
+
+// The y faces of every active strip fused with the residual and stage combine (the
+// single path's k_yflux_update, one row per step).  Mixed form: a cell next to a y face
+// whose FP32 states are not admissible is left to k_batch_fix (which recomputes its four
+// faces).  The next step's wave-speed maxima come from k_batch_speeds_all afterwards:
+// folding them in here (and loading the x fluxes ahead of the face arithmetic, as the
+// single path does) would cost this kernel 18-22 VGPRs each and its fourth wave/SIMD.
+template <int ORDER, bool CHAR, bool MIXED>
+__global__ __launch_bounds__(64) void k_batch_ysweep(BatchArgs a, int stage, int kind, int yi,
+                                                     int oi) {
+    const int lane = threadIdx.x;
+    const int nact = a.sched->nact, total = a.sched->total;
+    unsigned* queue = &a.sched->queue[stage];
+    for (int trip = 0; trip <= total; trip++) {
+        const unsigned ticket = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const int item = uniform((int)ticket);
+        if (item >= total) break;
+        const int ai = uniform(owner(a, nact, item));
+        const int k = uniform(a.act_prob[ai]);
+        const int local = item - a.act_start[ai];
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const int solver = d.solver;
+        const double dt = a.stat[k].dt;
+        const double* Y = a.pool[yi] + d.off;
+        const double* Un = a.pool[0] + d.off;
+        double* out = a.pool[oi] + d.off;
+        const double* F = a.flux + d.off;
+        const int sx = local / d.strips_y, sy = local - sx * d.strips_y;
+        const int i0 = GC + sx * a.rows;
+        const int i1 = min(i0 + a.rows, g.R - GC);
+        const int jf = GC + sy * kStripCols - 1 + lane;   // this lane's y face G[i][jf]
+        const int c = jf + 1;                             // this lane's cell column
+        const bool face_ok = jf <= g.C - GC - 1;
+        const bool cell_ok = lane < kStripCols && c <= g.C - GC - 1;
+        int fault = 0;
+        auto row_phys = [&]() {
+            const BProb* dq = a.probs + k;
+            asm volatile("" : "+s"(dq));
+            return dq->ph;
+        };
+        for (int i = i0; i < i1; i++) {
+            size_t ld = g.ld, pl = g.plane;
+            asm volatile("" : "+v"(ld), "+v"(pl));
+            const size_t row = (size_t)opaque_row(i) * ld;
+            double gf[4] = {0.0, 0.0, 0.0, 0.0};
+            int bad_g = 0;
+            if (face_ok) {
+                double st[6][4];
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * pl + row + jf - 2 + n];
+                const Phys pr = row_phys();
+                if constexpr (MIXED) {
+                    bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
+                } else {
+                    fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                }
+            }
+            double gu[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
+            const int bad_gu = __shfl_down(bad_g, 1, 64);
+            if (cell_ok) {
+                const size_t o0 = row + c;
+                if (MIXED && (bad_g || bad_gu)) {
+                    const unsigned q = atomicAdd(&a.sched->defer[stage], 1u);
+                    a.defer_list[q] = ((unsigned long long)k << 40) |
+                                      ((unsigned long long)i << 20) | (unsigned long long)c;
+                } else {
+                    double y[4], un[4] = {0.0, 0.0, 0.0, 0.0}, fl[4], fu[4], r[4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        y[v] = Y[(size_t)v * pl + o0];
+                        fu[v] = F[(size_t)v * pl + o0];
+                        fl[v] = F[(size_t)v * pl + o0 - ld];
+                        if (kind != UPD_EULER) un[v] = Un[(size_t)v * pl + o0];
+                    }
+                    const Phys pc = row_phys();
+                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) out[(size_t)v * pl + o0] = r[v];
+                }
+            }
+        }
+        if (!MIXED) {
+            const unsigned long long any = __ballot(fault != 0);
+            if (any && lane == 0) {
+                atomicOr(a.faults, 1u);
+                atomicOr(&a.stat[k].faults, 1u);
+            }
+        }
+    }
+}
+
+// Wave-speed maxima of every active problem's new state (the two-sweep schedule's last
+// stage): grid (blocks, problems); the same per-cell arithmetic and NaN-ignoring max as
+// k_dt_partial.
+__global__ __launch_bounds__(256) void k_batch_speeds_all(BatchArgs a) {
+    for (int k = blockIdx.y; k < a.nprob; k += gridDim.y) {
+        if (!a.stat[k].active) continue;
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const double* U = a.pool[0] + d.off;
+        double sx = 0.0, sy = 0.0;
+        const long total = (long)d.nx * d.ny;
+        for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+             t += (long)gridDim.x * blockDim.x) {
+            const int i = GC + (int)(t / d.ny), j = GC + (int)(t % d.ny);
+            const size_t o = (size_t)i * g.ld + j;
+            double c[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) c[v] = U[(size_t)v * g.plane + o];
+            double lx, ly;
+            speeds(d.ph.gamma, c, lx, ly);
+            if (lx > sx) sx = lx;
+            if (ly > sy) sy = ly;
+        }
+        sx = wave_max_b(sx);
+        sy = wave_max_b(sy);
+        if ((threadIdx.x & 63) == 0) speed_max(a.stat[k], sx, sy);
+    }
+}
+
 // ------------------------------------------------------------------ fix-up
 // Mixed form: each deferred cell with its four faces recomputed (mixed arithmetic,
 // FP64 for a face whose FP32 states are not admissible) -- the single-problem mixed
@@ -573,7 +814,25 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
             last = 1;
         }
     }
-    if (l.staging == STAGING_LDS)
+    if (l.staging == STAGING_SWEEPS) {
+        // blocks cover the longest side of any problem; each side checks its own range
+        const dim3 gg((a.max_side + 63) / 64, 4, a.nprob < 65535 ? a.nprob : 65535);
+        for (int side = 0; side < 4; side++)
+            hipLaunchKernelGGL(k_batch_ghosts, gg, dim3(64), 0, s, a, side, yi);
+        hipLaunchKernelGGL((k_batch_xsweep<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s, a,
+                           stage, yi);
+        if constexpr (MIXED)
+            hipLaunchKernelGGL((k_batch_xfix<ORDER, CHAR>), dim3(256), dim3(256), 0, s, a, stage, yi);
+        hipLaunchKernelGGL((k_batch_ysweep<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s, a,
+                           stage, kind, yi, oi);
+        if constexpr (MIXED)
+            hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(1024), dim3(256), 0, s, a, stage,
+                               kind, yi, oi, 0);
+        // RK1 writes U1 and copies back afterwards: its maxima come after the copy
+        if (last && time_order != 1)
+            launch_batch_speeds_all(a, s);
+        return;
+    } else if (l.staging == STAGING_LDS)
         hipLaunchKernelGGL((k_batch_stage<ORDER, CHAR, MIXED, STAGING_LDS>), dim3(l.grid), dim3(64),
                            0, s, a, stage, kind, yi, oi, last);
     else
@@ -604,6 +863,11 @@ void launch_batch_copy_back(const BatchArgs& a, hipStream_t s) {
 
 void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a, k);
+}
+
+void launch_batch_speeds_all(const BatchArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_speeds_all, dim3(64, a.nprob < 65535 ? a.nprob : 65535), dim3(256),
+                       0, s, a);
 }
 
 void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s) {

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from overheatingproblem_amd import cases
-from overheatingproblem_amd.capi import STAGING_DIRECT, STAGING_LDS, Batch, Solver
+from overheatingproblem_amd.capi import STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS, Batch, Solver
 from overheatingproblem_amd.config import (BC_FIXED_VALUE as BC_FIXED, BC_INFLOW, BC_OUTFLOW,
                                            BC_PERIODICITY as BC_PERIODIC,
                                            BC_REFLECTION as BC_REFLECT, PRECISION_MIXED)
@@ -57,12 +57,14 @@ def small_batch(precision=0, characteristic=True, seed=21):
     return cfgs, states
 
 
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
 @pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 @pytest.mark.parametrize("characteristic", [True, False])
-def test_batch_matches_single_problems(precision, characteristic):
+def test_batch_matches_single_problems(precision, characteristic, staging):
     cfgs, states = small_batch(precision, characteristic)
     t_end = 0.05
     with Batch(cfgs) as b:
+        b.set_tuning(staging=staging)
         for k, U in enumerate(states):
             b.set_state(k, U)
         b.advance_to(t_end, 100000)
@@ -77,7 +79,7 @@ def test_batch_matches_single_problems(precision, characteristic):
     bad = [r for r in report if r[5] != r[6] or r[7] != r[8] or r[9] != t_end or r[10] != 0.0]
     assert not bad, "\n".join(map(str, bad))
 
-@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_LDS])
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS])
 @pytest.mark.parametrize("rows,waves", [(4, 1), (16, 8), (64, 3)])
 def test_batch_launch_shapes_agree(staging, rows, waves):
     cfgs, states = small_batch(0, True, seed=11)
@@ -94,14 +96,15 @@ def test_batch_launch_shapes_agree(staging, rows, waves):
         assert np.array_equal(a, c)
 
 
-def test_batch_graph_replay_matches_eager():
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
+def test_batch_graph_replay_matches_eager(staging):
     """Graph replays with host work and synchronisation in between give the eager
     launches' results at every step."""
     cfgs, states = small_batch(0, True, seed=3)
     res = {}
     for use_graph in (0, 1):
         with Batch(cfgs) as b:
-            b.set_tuning(use_graph=use_graph)
+            b.set_tuning(use_graph=use_graph, staging=staging)
             for k, U in enumerate(states):
                 b.set_state(k, U)
             snaps = []
@@ -123,7 +126,7 @@ def test_batch_tuner_keeps_state_and_picks_a_valid_shape():
             b.set_state(k, U)
         b.step(t_end)
         tuned = b.tune()
-        assert tuned["ms_per_stage"] > 0 and tuned["staging"] in (0, 1)
+        assert tuned["ms_per_stage"] > 0 and tuned["staging"] in (0, 1, 2)
         b.advance_to(t_end, 100000)
         got = [interior(b.get_state(k)) for k in range(len(cfgs))]
     with Batch(cfgs) as b:
@@ -135,12 +138,14 @@ def test_batch_tuner_keeps_state_and_picks_a_valid_shape():
         assert np.array_equal(a, c)
 
 
-def test_batch_rk_orders():
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
+def test_batch_rk_orders(staging):
     for order in (1, 2):
         cfgs, states = small_batch(0, True, seed=order)
         for c in cfgs:
             c.time_order = order
         with Batch(cfgs) as b:
+            b.set_tuning(staging=staging)
             for k, U in enumerate(states):
                 b.set_state(k, U)
             b.advance_to(0.01, 100000)
@@ -150,8 +155,9 @@ def test_batch_rk_orders():
             assert np.array_equal(interior(got[k]), interior(ref)), (order, k)
 
 
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
 @pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
-def test_batch_seeded_workload_prefix(precision):
+def test_batch_seeded_workload_prefix(precision, staging):
     """The first problems of the seeded 64-problem workload (sizes 256..1536), a few
     steps: identical to the single-problem path."""
     cfgs, states = cases.batch_case(count=64)
@@ -161,6 +167,7 @@ def test_batch_seeded_workload_prefix(precision):
     for c in cfgs:
         c.precision = precision
     with Batch(cfgs) as b:
+        b.set_tuning(staging=staging)
         for k, U in enumerate(states):
             b.set_state(k, U)
         for _ in range(4):
