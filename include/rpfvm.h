@@ -144,7 +144,10 @@ int rpfvm_set_state(rpfvm_ctx *ctx, const double *U);
 int rpfvm_get_state(rpfvm_ctx *ctx, double *U);
 
 /* Get_Delta_T_2D, include/function.h:144-175: dt = CFL*min(dx,dy)/(max(|u|+a)+max(|v|+a))
- * over the interior.  Synchronises the stream and returns dt on the host. */
+ * over the interior (NaN speeds skipped).  After rpfvm_time_advance the maxima come from
+ * the last stage's kernel (the state is not read again); after rpfvm_set_state or
+ * rpfvm_device_state, from a pass over the state.  Both give the same dt, bit for bit.
+ * Synchronises the stream and returns dt on the host. */
 int rpfvm_get_delta_t(rpfvm_ctx *ctx, double *dt);
 
 /* RK1_TimeAd / RK2_TimeAd / RK3_TimeAd (include/time_advance.h:643-882), chosen by
@@ -193,7 +196,10 @@ int rpfvm_get_faults(rpfvm_ctx *ctx, int *faults);
 
 /* Device access for callers that keep data on the GPU.  The state lives in four
  * planes of R rows with a row pitch of *ld doubles (ld >= C); plane v starts at
- * base + v * plane_stride.  The stream is a hipStream_t. */
+ * base + v * plane_stride.  The stream is a hipStream_t.  The next dt is normally
+ * reduced from wave-speed maxima the last step left behind; calling
+ * rpfvm_device_state makes the next rpfvm_get_delta_t read the state itself, so call
+ * it again after writing the state through the pointer. */
 int rpfvm_device_state(rpfvm_ctx *ctx, double **base, size_t *ld, size_t *plane_stride);
 int rpfvm_stream(rpfvm_ctx *ctx, void **stream);
 

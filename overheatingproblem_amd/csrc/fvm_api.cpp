@@ -57,6 +57,8 @@ struct rpfvm_ctx {
     FaceFix fix{};                 // mixed form: faces recomputed in FP64
     unsigned stamp = 0;            // stage stamp of the safeguard
     double* partial = nullptr;
+    DtSlots slots{};               // last stage's wave-speed maxima (folded CFL reduction)
+    bool slots_valid = false;      // the slots describe the current state U
     double* scal = nullptr;        // [0] dt, [1] t, [2] t_stop
     unsigned* faults = nullptr;
     double* host_scal = nullptr;   // pinned mirror of scal
@@ -229,7 +231,8 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     const size_t rep_bytes = c.positivity ? 5 * cell_bytes + 256 : 0;
     const size_t face_bytes = (((size_t)(c.nx + 1) * (c.ny + 1) * 4 + 255) / 256) * 256;
     const size_t fix_bytes = c.precision == RPFVM_PRECISION_MIXED ? 2 * face_bytes + 256 : 0;
-    const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes;
+    const size_t slot_bytes = (((size_t)dt_slot_count(x->g) * 2 * sizeof(unsigned long long) + 255) / 256) * 256;
+    const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes + slot_bytes;
     he = hipMalloc(&x->pool, total);
     if (he != hipSuccess)
         return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(he)));
@@ -258,6 +261,8 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
         x->fix.list[0] = reinterpret_cast<int*>(r + 256);
         x->fix.list[1] = reinterpret_cast<int*>(r + 256 + face_bytes);
     }
+    x->slots.slot = reinterpret_cast<unsigned long long*>(q + small + rep_bytes + fix_bytes);
+    x->slots.nstrips = (c.ny + kStrip - 1) / kStrip;
     he = hipMemsetAsync(x->pool, 0, total, x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
     he = hipHostMalloc(reinterpret_cast<void**>(&x->host_scal), 256, hipHostMallocDefault);
@@ -302,6 +307,7 @@ static int copy_out(rpfvm_ctx* x, double* host, const double* dev) {
 
 int rpfvm_set_state(rpfvm_ctx* x, const double* U) {
     if (!x || !U) return fail(RPFVM_ERR_INVALID, "null argument");
+    x->slots_valid = false;
     return copy_in(x, x->U, U);
 }
 
@@ -338,24 +344,32 @@ static void enqueue_update(rpfvm_ctx* x, const double* Un, const double* Y, doub
 
 static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
     Timed t(x, RPFVM_KCLASS_DT);
-    launch_dt(x->U, x->g, x->ph, x->partial, x->scal, clock, x->faults, x->stream);
+    if (x->slots_valid)
+        launch_dt_slots(x->U, x->g, x->ph, x->slots, x->partial, x->scal, clock, x->faults,
+                        x->stream);
+    else
+        launch_dt(x->U, x->g, x->ph, x->partial, x->scal, clock, x->faults, x->stream);
 }
 
 // One stage: ghost fill of Y, x-face fluxes, then the y faces fused with the residual
 // and stage combine into out (separately when out aliases Y, which the fused kernel's
 // neighbour reads forbid), then the positivity safeguard.
-static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
-                          int kind, double t_stage) {
+// `last`: the step's last stage, which leaves the next step's wave-speed slots (fused
+// path only; returns whether it did).
+static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
+                          int kind, double t_stage, bool last) {
     const unsigned stamp = ++x->stamp;
     enqueue_bc(x, Y, t_stage);
     {
         Timed t(x, RPFVM_KCLASS_FLUX_X);
         x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
     }
-    if (out != Y) {
+    const bool fused = out != Y;
+    const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
+    if (fused) {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
-        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, x->faults,
-                   x->stream);
+        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds,
+                   x->faults, x->stream);
     } else {
         {
             Timed t(x, RPFVM_KCLASS_FLUX_Y);
@@ -365,9 +379,10 @@ static void enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
     }
     if (x->rp.cnt) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
-        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, x->faults,
+        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
                       x->stream);
     }
+    return ds.slot != nullptr;
 }
 
 static int enqueue_step(rpfvm_ctx* x, double dt) {
@@ -378,20 +393,22 @@ static int enqueue_step(rpfvm_ctx* x, double dt) {
     double* last = x->U3 ? x->U3 : x->U;
     // stage k fills its ghosts at t + c_k dt (SSP-RK3: c = 0, 1, 1/2; RK2: 0, 1)
     const double t = x->time;
+    bool slots = false;
     switch (x->cfg.time_order) {
         case 1:
-            enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER, t);
+            slots = enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER, t, true);
             break;
         case 2:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t);
-            enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2, t + dt);
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t, false);
+            slots = enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2, t + dt, true);
             break;
         default:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t);
-            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2, t + dt);
-            enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3, t + 0.5 * dt);
+            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t, false);
+            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2, t + dt, false);
+            slots = enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3, t + 0.5 * dt, true);
             break;
     }
+    x->slots_valid = slots;
     x->time = t + dt;
     if (last != x->U) {
         launch_copy_frame(x->U, last, x->g, x->stream);
@@ -542,6 +559,7 @@ int rpfvm_kernel_times(rpfvm_ctx* x, double* ms, int* counts) {
 
 int rpfvm_device_state(rpfvm_ctx* x, double** base, size_t* ld, size_t* plane_stride) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
+    x->slots_valid = false;  // the caller may write the state: next dt from the state itself
     if (base) *base = x->U;
     if (ld) *ld = (size_t)x->g.ld;
     if (plane_stride) *plane_stride = x->g.plane;

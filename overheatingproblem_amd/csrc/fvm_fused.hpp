@@ -14,7 +14,6 @@
 
 namespace fvm {
 
-constexpr int kStrip = 63;  // cells updated per 64-lane row
 
 // Block (x, y) of a 2-D grid, remapped so that each XCD works on one contiguous range of
 // the grid.  Workgroups are dispatched to the 8 XCDs round-robin by linear id, and each
@@ -32,6 +31,24 @@ __device__ __forceinline__ Tile2 xcd_tile() {
     const unsigned per = n / 8;
     const unsigned l = lin < per * 8 ? (lin % 8) * per + lin / 8 : lin;
     return Tile2{(int)(l % nx), (int)(l / nx)};
+}
+
+// |u| + a and |v| + a of one cell, formed as Get_Delta_T_2D does (include/function.h:
+// 144-175); the CFL reduction's max ignores NaN like the reference's `if (s > S) S = s`.
+__device__ __forceinline__ void cell_speeds(double gam, double rho, double ru, double rv,
+                                            double E, double& lx, double& ly) {
+    const double u = ru / rho;
+    const double v = rv / rho;
+    const double p = (E - 0.5 * rho * (u * u + v * v)) * (gam - 1.0);
+    const double a = sqrt(gam * p / rho);
+    lx = fabs(u) + a;
+    ly = fabs(v) + a;
+}
+__device__ __forceinline__ double nanmax(double a, double b) { return b > a ? b : a; }
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = nanmax(x, __shfl_xor(x, o, 64));
+    return x;
 }
 
 // Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751)
@@ -81,7 +98,7 @@ __device__ __forceinline__ void cell_update_regs(const double* Un, double* out,
                                                  const double* __restrict__ F, const Geom& g,
                                                  const Phys& ph, double dt, int kind, int i,
                                                  int c, const double y[4], const double gl[4],
-                                                 const double gu[4], double W[4]) {
+                                                 const double gu[4], double W[4], double R[4]) {
     const size_t o0 = (size_t)i * g.ld + c;
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
@@ -104,6 +121,7 @@ __device__ __forceinline__ void cell_update_regs(const double* Un, double* out,
             default: r = (1.0 / 3.0) * Un[o] + (2.0 / 3.0) * (y[k] + dt * L); break;
         }
         out[o] = r;
+        R[k] = r;
     }
 }
 
@@ -114,7 +132,7 @@ __device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, cons
                                                 const double y[4], const double un[4],
                                                 const double fl[4], const double fu[4],
                                                 const double gl[4], const double gu[4],
-                                                double W[4]) {
+                                                double W[4], double R[4]) {
     const size_t o0 = (size_t)i * g.ld + c;
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
@@ -136,6 +154,7 @@ __device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, cons
             default: r = (1.0 / 3.0) * un[k] + (2.0 / 3.0) * (y[k] + dt * L); break;
         }
         out[(size_t)k * g.plane + o0] = r;
+        R[k] = r;
     }
 }
 
@@ -148,7 +167,8 @@ template <bool PREFETCH, typename FaceOp>
 __device__ __forceinline__ void yflux_update_body(const double* Un, const double* Y, double* out,
                                                   const double* __restrict__ F, double* G,
                                                   const Geom& g, const Phys& ph, double dt,
-                                                  int kind, const Repair& rp, FaceOp& op) {
+                                                  int kind, const Repair& rp, const DtSlots& ds,
+                                                  FaceOp& op) {
     const int t = threadIdx.x;
     const Tile2 b = xcd_tile();
     const int i = GC + b.y * blockDim.y + threadIdx.y;
@@ -188,21 +208,40 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
 #pragma unroll
     for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
     const int pending_up = __shfl_down(pending, 1, 64);
+    // last stage: the cell's wave speeds for the next step's dt, "if (s > S) S = s" from
+    // S = 0 in each lane (a NaN speed is skipped, as in the full pass and Get_Delta_T_2D,
+    // instead of hiding the row's other maxima)
+    double lx = 0.0, ly = 0.0;
     if (cell_ok) {
         // the cell's stage input, read again (a cache hit) rather than kept live in
         // registers across the face arithmetic
         double y[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) y[v] = Y[(size_t)v * g.plane + (size_t)i * g.ld + c];
-        double W[4];
-        if constexpr (PREFETCH) cell_update_pre(out, g, ph, dt, kind, i, c, y, un, fl, fu, gf, gu, W);
-        else cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu, W);
+        double W[4], R[4];
+        if constexpr (PREFETCH) cell_update_pre(out, g, ph, dt, kind, i, c, y, un, fl, fu, gf, gu, W, R);
+        else cell_update_regs(Un, out, F, g, ph, dt, kind, i, c, y, gf, gu, W, R);
+        if (ds.slot) {
+            double sx, sy;
+            cell_speeds(ph.gamma, R[0], R[1], R[2], R[3], sx, sy);
+            if (sx > lx) lx = sx;
+            if (sy > ly) ly = sy;
+        }
         if (rp.cnt && !pending && !pending_up) {
             const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
             if (!admissible(kc, W)) {
                 const unsigned k = atomicAdd(&rp.cnt[0], 1u);
                 rp.list[0][k] = (i - GC) * rp.ny + (c - GC);
             }
+        }
+    }
+    if (ds.slot) {
+        lx = wave_max(lx);
+        ly = wave_max(ly);
+        if (t == 0 && i < g.R - GC) {
+            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((f + 1 - GC) / kStrip);
+            ds.slot[2 * w] = (unsigned long long)__double_as_longlong(lx);
+            ds.slot[2 * w + 1] = (unsigned long long)__double_as_longlong(ly);
         }
     }
 }

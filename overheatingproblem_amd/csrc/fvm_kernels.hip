@@ -12,6 +12,8 @@
 // lanes walk the contiguous (y) index, so every load and store is coalesced.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fvm_bc.hpp"
 #include "fvm_fused.hpp"
 #include "fvm_kernels.hpp"
@@ -108,10 +110,10 @@ template <int ORDER, bool CHAR, int SOLVER>
 __global__ FUSED_BOUNDS void k_yflux_update(const double* Un, const double* Y,
                                                       double* out, const double* __restrict__ F,
                                                       double* G, Geom g, Phys ph, double dt,
-                                                      int kind, Repair rp,
+                                                      int kind, Repair rp, DtSlots ds,
                                                       unsigned* __restrict__ faults) {
     FaceOp64<ORDER, CHAR, SOLVER> op{ph, make_consts(ph), faults};
-    yflux_update_body<false>(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
+    yflux_update_body<false>(Un, Y, out, F, G, g, ph, dt, kind, rp, ds, op);
 }
 
 // ------------------------------------------------------------------ update
@@ -198,14 +200,18 @@ __device__ __forceinline__ void repair_cells_phase(const double* Un, const doubl
                                                    const double* __restrict__ G, const Geom& g,
                                                    const Phys& ph, double dt, int kind,
                                                    const Repair& rp, int round, unsigned stamp,
-                                                   unsigned* __restrict__ faults, unsigned b,
-                                                   unsigned nb) {
+                                                   const DtSlots& ds, unsigned* __restrict__ faults,
+                                                   unsigned b, unsigned nb) {
     const int nxt = (round & 1) ^ 1;
     const unsigned n = rp.cnt[2];
     for (unsigned q = b * blockDim.x + threadIdx.x; q < n; q += nb * blockDim.x) {
         const int c = rp.touched[q];
         const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, GC + c / rp.ny,
                                     GC + c % rp.ny);
+        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
+            const size_t w = (size_t)(c / rp.ny) * ds.nstrips + (size_t)((c % rp.ny) / kStrip);
+            ds.slot[2 * w] = kSlotStale;
+        }
         if (!ok) {
             if (rp.rep[c] != stamp) {
                 const unsigned k = atomicAdd(&rp.cnt[nxt], 1u);
@@ -232,10 +238,10 @@ __global__ __launch_bounds__(256) void k_repair_cells(const double* Un, const do
                                                       const double* __restrict__ F,
                                                       const double* __restrict__ G, Geom g,
                                                       Phys ph, double dt, int kind, Repair rp,
-                                                      int round, unsigned stamp,
+                                                      int round, unsigned stamp, DtSlots ds,
                                                       unsigned* __restrict__ faults) {
-    repair_cells_phase(Un, Y, out, F, G, g, ph, dt, kind, rp, round, stamp, faults, blockIdx.x,
-                       gridDim.x);
+    repair_cells_phase(Un, Y, out, F, G, g, ph, dt, kind, rp, round, stamp, ds, faults,
+                       blockIdx.x, gridDim.x);
 }
 
 // After the last round: cells still listed were never repaired; reset the counters.
@@ -278,7 +284,7 @@ __device__ __forceinline__ void repair_grid_sync(unsigned* bar, unsigned target,
 __global__ __launch_bounds__(256) void k_repair_all(const double* Un, const double* Y, double* out,
                                                     double* F, double* G, Geom g, Phys ph,
                                                     double dt, int kind, Repair rp, unsigned stamp,
-                                                    unsigned* __restrict__ faults) {
+                                                    DtSlots ds, unsigned* __restrict__ faults) {
     if (__hip_atomic_load(&rp.cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
     const unsigned b = blockIdx.x, nb = gridDim.x;
     unsigned* bar = &rp.cnt[8];
@@ -290,7 +296,7 @@ __global__ __launch_bounds__(256) void k_repair_all(const double* Un, const doub
         repair_grid_sync(bar, ++passed * nb, faults);
         repair_touch_phase(rp, r, stamp * 16u + (unsigned)r, b, nb);
         repair_grid_sync(bar, ++passed * nb, faults);
-        repair_cells_phase(Un, Y, out, F, G, g, ph, dt, kind, rp, r, stamp, faults, b, nb);
+        repair_cells_phase(Un, Y, out, F, G, g, ph, dt, kind, rp, r, stamp, ds, faults, b, nb);
         repair_grid_sync(bar, ++passed * nb, faults);
     }
     if (threadIdx.x == 0 && atomicAdd(&rp.cnt[9], 1u) == nb - 1) {
@@ -329,13 +335,6 @@ __global__ void k_copy_frame(const double* __restrict__ src, double* __restrict_
 // Get_Delta_T_2D (include/function.h:144-175): S_x = max(|u|+a), S_y = max(|v|+a)
 // over the interior; "if (s > S) S = s" ignores NaN exactly as the reference.  The
 // max is exact, so the block order does not matter.
-__device__ __forceinline__ double nanmax(double a, double b) { return b > a ? b : a; }
-
-__device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x = nanmax(x, __shfl_xor(x, o, 64));
-    return x;
-}
 
 __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U, Geom g, Phys ph,
                                                     double* __restrict__ partial) {
@@ -347,12 +346,8 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
          t += (long)gridDim.x * blockDim.x) {
         const int i = GC + (int)(t / nj), j = GC + (int)(t % nj);
         const size_t c = (size_t)i * g.ld + j;
-        const double rho = U[c];
-        const double u = U[g.plane + c] / rho;
-        const double v = U[2 * g.plane + c] / rho;
-        const double p = (U[3 * g.plane + c] - 0.5 * rho * (u * u + v * v)) * (gam - 1.0);
-        const double a = sqrt(gam * p / rho);
-        const double lx = fabs(u) + a, ly = fabs(v) + a;
+        double lx, ly;
+        cell_speeds(gam, U[c], U[g.plane + c], U[2 * g.plane + c], U[3 * g.plane + c], lx, ly);
         if (lx > sx) sx = lx;
         if (ly > sy) sy = ly;
     }
@@ -363,6 +358,62 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
     if (lane == 0) {
         red[0][w] = sx;
         red[1][w] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0][0], b = red[1][0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+            a = nanmax(a, red[0][k]);
+            b = nanmax(b, red[1][k]);
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// The same partial maxima from the slots of the last stage.  Each wavefront takes 64
+// consecutive slots; a stale one (kSlotStale) has its row segment of up to 63 cells
+// recomputed from U by the whole wavefront, one cell per lane.
+__global__ __launch_bounds__(256) void k_dt_slots(const double* __restrict__ U, Geom g, Phys ph,
+                                                  DtSlots ds, double* __restrict__ partial) {
+    const double gam = ph.gamma;
+    double sx = 0.0, sy = 0.0;
+    const int nj = g.C - 2 * GC;
+    const int lane = threadIdx.x & 63;
+    const long total = (long)(g.R - 2 * GC) * ds.nstrips;
+    const long waves = (long)gridDim.x * (blockDim.x >> 6);
+    for (long base = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < total;
+         base += waves * 64) {
+        const long w = base + lane;
+        bool stale = false;
+        if (w < total) {
+            const unsigned long long bx = ds.slot[2 * w], by = ds.slot[2 * w + 1];
+            stale = bx == kSlotStale;
+            if (!stale) {  // slots hold maxima from 0 with NaN skipped: never NaN
+                sx = nanmax(sx, __longlong_as_double((long long)bx));
+                sy = nanmax(sy, __longlong_as_double((long long)by));
+            }
+        }
+        for (unsigned long long m = __ballot(stale); m; m &= m - 1) {
+            const long ws = base + (__ffsll((long long)m) - 1);
+            const int i = GC + (int)(ws / ds.nstrips);
+            const int j = (int)(ws % ds.nstrips) * kStrip + lane;
+            if (lane < kStrip && j < nj) {
+                const size_t c = (size_t)i * g.ld + GC + j;
+                double lx, ly;
+                cell_speeds(gam, U[c], U[g.plane + c], U[2 * g.plane + c], U[3 * g.plane + c], lx, ly);
+                if (lx > sx) sx = lx;
+                if (ly > sy) sy = ly;
+            }
+        }
+    }
+    sx = wave_max(sx);
+    sy = wave_max(sy);
+    __shared__ double red[2][4];
+    const int wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wv] = sx;
+        red[1][wv] = sy;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -415,10 +466,10 @@ static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& p
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_yflux_update(const double* Un, const double* Y, double* out, const double* F,
                                 double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                                const Repair& rp, const FaceFix&, unsigned, unsigned* faults,
-                                hipStream_t s) {
+                                const Repair& rp, const FaceFix&, unsigned, const DtSlots& ds,
+                                unsigned* faults, hipStream_t s) {
     hipLaunchKernelGGL((k_yflux_update<ORDER, CHAR, SOLVER>), yflux_update_grid(g), yflux_update_block(), 0,
-                       s, Un, Y, out, F, G, g, ph, dt, kind, rp, faults);
+                       s, Un, Y, out, F, G, g, ph, dt, kind, rp, ds, faults);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
@@ -502,7 +553,7 @@ void launch_update(const double* Un, const double* Y, double* out, const double*
 
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
                    const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
-                   unsigned stamp, unsigned* faults, hipStream_t s) {
+                   unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
     const dim3 block(256), grid(64);
     {
         // one cooperative launch (all 64 blocks resident); the separate launches below
@@ -510,9 +561,10 @@ void launch_repair(const double* Un, const double* Y, double* out, double* F, do
         Geom gg = g;
         Phys pp = ph;
         Repair rr = rp;
+        DtSlots dd = ds;
         void* args[] = {(void*)&Un, (void*)&Y, (void*)&out, (void*)&F, (void*)&G, (void*)&gg,
                         (void*)&pp, (void*)&dt, (void*)&kind, (void*)&rr, (void*)&stamp,
-                        (void*)&faults};
+                        (void*)&dd, (void*)&faults};
         if (hipLaunchCooperativeKernel((const void*)k_repair_all, grid, block, args, 0, s) ==
             hipSuccess)
             return;
@@ -522,7 +574,7 @@ void launch_repair(const double* Un, const double* Y, double* out, double* F, do
         hipLaunchKernelGGL(k_repair_faces, grid, block, 0, s, Y, F, G, g, ph, rp, r, stamp, faults);
         hipLaunchKernelGGL(k_repair_touch, grid, block, 0, s, rp, r, stamp * 16u + (unsigned)r);
         hipLaunchKernelGGL(k_repair_cells, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp,
-                           r, stamp, faults);
+                           r, stamp, ds, faults);
     }
     hipLaunchKernelGGL(k_repair_finish, dim3(1), dim3(64), 0, s, rp, kRepairRounds & 1, faults);
 }
@@ -536,6 +588,19 @@ int dt_blocks(const Geom& g) {
     const long cells = (long)(g.R - 2 * GC) * (g.C - 2 * GC);
     long b = (cells + 255) / 256;
     return (int)(b < 2048 ? (b < 1 ? 1 : b) : 2048);
+}
+
+int dt_slot_count(const Geom& g) {
+    return (g.R - 2 * GC) * ((g.C - 2 * GC + kStrip - 1) / kStrip);
+}
+
+void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
+                     double* partial, double* dt_out, const double* clock, unsigned* faults,
+                     hipStream_t s) {
+    const long slots = dt_slot_count(g);
+    const int nb = (int)std::min<long>(dt_blocks(g), std::max<long>(1, (slots + 255) / 256));
+    hipLaunchKernelGGL(k_dt_slots, dim3(nb), dim3(256), 0, s, U, g, ph, ds, partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, partial, nb, ph, dt_out, clock, faults);
 }
 
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,

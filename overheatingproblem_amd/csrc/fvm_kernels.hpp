@@ -90,6 +90,20 @@ struct FaceFix {
     int* list[2];   // face offsets i*ld + j
 };
 
+// Wave-speed maxima of the last stage's new state, one slot per wavefront row of the fused
+// y-flux kernel: slot w = (i - GC) * nstrips + (j - GC) / 63 holds the bit patterns of
+// max(|u|+a) and max(|v|+a) over cells (i, GC + 63 s .. GC + 63 s + 62) (non-negative
+// doubles order as unsigned integers).  A kernel that changes such a cell afterwards
+// marks its slot stale with the all-ones pattern; the reduction recomputes stale slots
+// from the state.  slot == nullptr: not the last stage.
+constexpr int kStrip = 63;  // cells updated per 64-lane row of the fused y-flux kernels
+
+struct DtSlots {
+    unsigned long long* slot;
+    int nstrips;
+};
+constexpr unsigned long long kSlotStale = ~0ull;
+
 using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phys& ph,
                             unsigned* faults, const FaceFix& fix, hipStream_t s);
 // y faces fused with the residual + stage combine (out must not alias Y); `stamp` is
@@ -97,7 +111,7 @@ using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phy
 using FusedLaunch = void (*)(const double* Un, const double* Y, double* out, const double* F,
                              double* G, const Geom& g, const Phys& ph, double dt, int kind,
                              const Repair& rp, const FaceFix& fix, unsigned stamp,
-                             unsigned* faults, hipStream_t s);
+                             const DtSlots& ds, unsigned* faults, hipStream_t s);
 struct FluxPair {
     FluxLaunch x, y;
     FusedLaunch yu;
@@ -114,14 +128,20 @@ void launch_update(const double* Un, const double* Y, double* out, const double*
                    const Repair& rp, hipStream_t s);
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
                    const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
-                   unsigned stamp, unsigned* faults, hipStream_t s);
+                   unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
 // Cells next to the y faces the mixed form recomputed in FP64 (fvm_mixed_kernels.hip).
 void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
                        const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                       const Repair& rp, const FaceFix& fix, unsigned stamp, hipStream_t s);
+                       const Repair& rp, const FaceFix& fix, unsigned stamp, const DtSlots& ds,
+                       hipStream_t s);
 void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s);
 int dt_blocks(const Geom& g);
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,
                const double* clock, unsigned* faults, hipStream_t s);
+// The same dt from the slots the last stage left (stale slots recomputed from U).
+void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
+                     double* partial, double* dt_out, const double* clock, unsigned* faults,
+                     hipStream_t s);
+int dt_slot_count(const Geom& g);
 
 }  // namespace fvm

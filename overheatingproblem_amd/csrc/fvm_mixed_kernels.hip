@@ -102,14 +102,18 @@ struct FaceOpMx {
     }
 };
 
-template <int ORDER, bool CHAR, int SOLVER>
+// SLOTS: the last stage, which also leaves the next step's wave-speed maxima in `ds`.
+// A kernel of its own, so that the earlier stages keep the register budget of 4
+// waves/SIMD (the speeds' divisions and square root add 7 VGPRs).
+template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
 __global__ FUSED_MX_BOUNDS void k_yflux_update_mixed(const double* Un, const double* Y,
                                                             double* out,
                                                             const double* __restrict__ F,
                                                             double* G, Geom g, Phys ph, double dt,
-                                                            int kind, Repair rp, FaceFix fix) {
+                                                            int kind, Repair rp, FaceFix fix,
+                                                            DtSlots ds) {
     FaceOpMx<ORDER, CHAR, SOLVER> op{ph, fix};
-    yflux_update_body<true>(Un, Y, out, F, G, g, ph, dt, kind, rp, op);
+    yflux_update_body<true>(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, op);
 }
 
 // Cells on either side of each recomputed y face: residual + stage combine again, with
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const doub
                                                     const double* __restrict__ F,
                                                     const double* __restrict__ G, Geom g, Phys ph,
                                                     double dt, int kind, Repair rp, FaceFix fix,
-                                                    unsigned tstamp) {
+                                                    unsigned tstamp, DtSlots ds) {
     const unsigned n = fix.cnt[1];
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n; q += gridDim.x * blockDim.x) {
         const size_t c0 = (size_t)fix.list[1][q >> 1];
@@ -127,6 +131,10 @@ __global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const doub
         const int j = (int)(c0 % g.ld) + (int)(q & 1);
         if (j < GC || j > g.C - GC - 1) continue;
         const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
+            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
+            ds.slot[2 * w] = kSlotStale;
+        }
         if (rp.cnt && !ok) {
             const int cell = (i - GC) * rp.ny + (j - GC);
             if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
@@ -139,21 +147,27 @@ __global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const doub
 
 void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
                        const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                       const Repair& rp, const FaceFix& fix, unsigned stamp, hipStream_t s) {
+                       const Repair& rp, const FaceFix& fix, unsigned stamp, const DtSlots& ds,
+                       hipStream_t s) {
     hipLaunchKernelGGL(k_yfix_cells, dim3(64), dim3(256), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
-                       rp, fix, stamp * 16u + 15u);
+                       rp, fix, stamp * 16u + 15u, ds);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_yflux_update_mixed(const double* Un, const double* Y, double* out,
                                       const double* F, double* G, const Geom& g, const Phys& ph,
                                       double dt, int kind, const Repair& rp, const FaceFix& fix,
-                                      unsigned stamp, unsigned* faults, hipStream_t s) {
-    hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER>), yflux_update_grid(g),
-                       yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix);
+                                      unsigned stamp, const DtSlots& ds, unsigned* faults,
+                                      hipStream_t s) {
+    if (ds.slot)
+        hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER, true>), yflux_update_grid(g),
+                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+    else
+        hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER, false>), yflux_update_grid(g),
+                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
     hipLaunchKernelGGL((k_flux_fix<2, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, G, g,
                        ph, fix, faults);
-    launch_yfix_cells(Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, s);
+    launch_yfix_cells(Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, s);
 }
 
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
