@@ -80,7 +80,13 @@ enum {
 };
 
 /* rpfvm_config.debug_flags bits. */
-enum { RPFVM_DEBUG_MIXED_ALL_FP64 = 1 };
+enum {
+    RPFVM_DEBUG_MIXED_ALL_FP64 = 1,  /* every mixed-form face through the FP64 recompute list */
+    RPFVM_DEBUG_ROW_KERNEL = 2       /* the fused y sweep (both forms) as one row per
+                                        wavefront (the form used for arrays of 4 GB and more)
+                                        instead of the LDS-staged marching form; identical
+                                        results */
+};
 
 /* Error codes (< 0). */
 enum {
@@ -112,9 +118,10 @@ typedef struct rpfvm_config {
                                   rho or p is not positive (or not finite) falls back to
                                   first-order (piecewise-constant) states.  Not in the
                                   reference, which has no positivity safeguard. */
-    int32_t debug_flags;       /* 0 normally.  RPFVM_DEBUG_MIXED_ALL_FP64: the mixed form
-                                  routes every face through its FP64 recompute list
-                                  (verification of that path only) */
+    int32_t debug_flags;       /* 0 normally; RPFVM_DEBUG_* bits select verification-only
+                                  paths (RPFVM_DEBUG_MIXED_ALL_FP64: the mixed form routes
+                                  every face through its FP64 recompute list;
+                                  RPFVM_DEBUG_ROW_KERNEL: see above) */
     int32_t bc[4];             /* left, right, bottom, top: RPFVM_BC_* */
     double dx, dy;
     double gamma, cfl, gravity;

@@ -126,14 +126,12 @@ __device__ __forceinline__ void cell_update_regs(const double* Un, double* out,
 }
 
 // cell_update_regs with the x fluxes (fl = F_{i-1/2}, fu = F_{i+1/2}) and U^n already
-// in registers (un is not read by UPD_EULER).
-__device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, const Phys& ph,
-                                                double dt, int kind, int i, int c,
-                                                const double y[4], const double un[4],
-                                                const double fl[4], const double fu[4],
-                                                const double gl[4], const double gu[4],
-                                                double W[4], double R[4]) {
-    const size_t o0 = (size_t)i * g.ld + c;
+// in registers (un is not read by UPD_EULER); the new state is left in R, not stored.
+__device__ __forceinline__ void cell_update_vals(const Phys& ph, double dt, int kind,
+                                                 const double y[4], const double un[4],
+                                                 const double fl[4], const double fu[4],
+                                                 const double gl[4], const double gu[4],
+                                                 double W[4], double R[4]) {
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
         const double rho = y[0];
@@ -153,9 +151,21 @@ __device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, cons
             case UPD_RK3_2: r = (3.0 / 4.0) * un[k] + (1.0 / 4.0) * (y[k] + dt * L); break;
             default: r = (1.0 / 3.0) * un[k] + (2.0 / 3.0) * (y[k] + dt * L); break;
         }
-        out[(size_t)k * g.plane + o0] = r;
         R[k] = r;
     }
+}
+
+// cell_update_vals, storing the new state of cell (i, c) to out.
+__device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, const Phys& ph,
+                                                double dt, int kind, int i, int c,
+                                                const double y[4], const double un[4],
+                                                const double fl[4], const double fu[4],
+                                                const double gl[4], const double gu[4],
+                                                double W[4], double R[4]) {
+    const size_t o0 = (size_t)i * g.ld + c;
+    cell_update_vals(ph, dt, kind, y, un, fl, fu, gl, gu, W, R);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[(size_t)k * g.plane + o0] = R[k];
 }
 
 // FaceOp::compute(st, f) returns false when the face must be recomputed later (mixed
@@ -247,6 +257,218 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
 }
 
 constexpr int kFusedRows = RPFVM_FUSED_ROWS;  // grid rows per workgroup of the fused kernels
+
+// ------------------------------------------------------------------ marching form
+// A wavefront takes kMarchRows consecutive rows of one 63-cell strip and walks down them.
+// While it computes row i, the stencil window of row i + 1 (cells f0-2 .. f0+66, 4
+// variables) is on its way into LDS by direct global->LDS loads (global_load_lds_dword:
+// each lane moves one dword and no VGPR holds it), and the x-flux and U^n rows of i + 1
+// follow as soon as row i's update has read its own.  A cell's lower x-flux row is the
+// upper row of the cell above, carried in registers.  The wait for those loads sits after
+// the face arithmetic and before the row's first store: by then they have arrived, and
+// no store of the row is outstanding yet.  The arithmetic and its inputs are
+// yflux_update_body's, so the results are identical.
+#ifndef RPFVM_MARCH_ROWS
+#define RPFVM_MARCH_ROWS 2
+#endif
+#ifndef RPFVM_MARCH_WAVES
+#define RPFVM_MARCH_WAVES 2
+#endif
+constexpr int kMarchRows = RPFVM_MARCH_ROWS;    // rows per wavefront
+constexpr int kMarchWaves = RPFVM_MARCH_WAVES;  // wavefronts per workgroup
+constexpr int kWinCells = kStrip + 6;  // stencil window of a row: cells f0-2 .. f0+66
+constexpr int kWinPitch = 96;          // doubles per variable of a window in LDS (3 x 64 dwords)
+constexpr int kRowPitch = 64;          // doubles per variable of an F or U^n row (2 x 64 dwords)
+// doubles of LDS per wavefront: two windows, one F row, one U^n row (10 KB: 16 wavefronts
+// fill the CU's 160 KB, 4 per SIMD)
+constexpr int kMarchLds = 2 * 4 * kWinPitch + 2 * 4 * kRowPitch;
+
+// Compiler barriers around the hand-placed waits: no memory access moves across them.
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void wait_lds() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// buffer_load_dword ... lds: LDS[M0 + 4 * lane] = dword at rsrc.base + voffset + soffset.
+__device__ void buffer_load_lds_dword(i32x4 rsrc, lds_void* lds, int size, int voffset, int soffset,
+                                      int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+__device__ void buffer_store_f64(double v, i32x4 rsrc, int voffset, int soffset,
+                                 int aux) __asm("llvm.amdgcn.raw.buffer.store.f64");
+
+// Buffer descriptor over `bytes` bytes from `base` (raw, stride 0; gfx9 word 3); loads
+// past `bytes` return zero.
+__device__ __forceinline__ i32x4 buffer_rsrc(const void* base, unsigned bytes) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+    i32x4 r;
+    r.x = __builtin_amdgcn_readfirstlane((int)(unsigned)a);
+    r.y = __builtin_amdgcn_readfirstlane((int)(unsigned)(a >> 32));
+    r.z = __builtin_amdgcn_readfirstlane((int)bytes);
+    r.w = 0x00020000;
+    return r;
+}
+
+// NI x 64 dwords into dst in LDS: dword 64k + t of dst gets the dword at byte
+// soffset + voff[k] of the buffer, where voff[k] = 4 min(64k + t, valid dwords - 1) (every
+// lane loads; the copies past the valid range are never read).
+template <int NI>
+__device__ __forceinline__ void lds_fetch(i32x4 rsrc, int soffset, double* dst, const int (&voff)[NI]) {
+    unsigned* d = reinterpret_cast<unsigned*>(dst);
+#pragma unroll
+    for (int k = 0; k < NI; k++) buffer_load_lds_dword(rsrc, (lds_void*)(d + 64 * k), 4, voff[k], soffset, 0, 0);
+}
+
+// The marching form addresses each input array through one buffer descriptor with 32-bit
+// offsets: the host uses it only when 4 planes fit in 4 GB (march_fits).
+inline bool march_fits(const Geom& g) { return (unsigned long long)g.plane * 4 * sizeof(double) < (1ull << 32); }
+
+template <typename FaceOp>
+__device__ __forceinline__ void yflux_update_march(const double* Un, const double* Y, double* out,
+                                                   const double* __restrict__ F, double* G,
+                                                   const Geom& g, const Phys& ph, double dt,
+                                                   int kind, const Repair& rp, const DtSlots& ds,
+                                                   FaceOp& op, double* lds) {
+    const int t = threadIdx.x;
+    const Tile2 b = xcd_tile();
+    // wave-uniform values in SGPRs (xcd_tile divides in the VALU): every address below is
+    // then an SGPR base plus a lane offset
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int i0 = __builtin_amdgcn_readfirstlane(GC + (b.y * kMarchWaves + wv) * kMarchRows);
+    const int i1 = min(i0 + kMarchRows, g.R - GC);
+    if (i0 >= i1) return;  // whole wavefront
+    double* const win = lds + wv * kMarchLds;      // [2][4][kWinPitch]
+    double* const frow = win + 2 * 4 * kWinPitch;  // [4][kRowPitch]
+    double* const urow = frow + 4 * kRowPitch;     // [4][kRowPitch]
+    const int f0 = __builtin_amdgcn_readfirstlane(GC - 1 + b.x * kStrip);
+    const int f_lane = f0 + t;
+    const bool face_ok = f_lane <= g.C - GC - 1;
+    const int c_lane = f_lane + 1;
+    const bool cell_ok = t < kStrip && face_ok && c_lane <= g.C - GC - 1;
+    // window cells f0-2 .. f0-2+ncell-1 (<= C-1); row cells f0+1 .. f0+nrow (<= C-GC-1)
+    const int win_words = 2 * min(kWinCells, g.C - (f0 - 2));
+    const int row_words = 2 * min(kStrip, g.C - GC - 1 - f0);
+    int woff[3], roff[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) woff[k] = 4 * min(64 * k + t, win_words - 1);
+#pragma unroll
+    for (int k = 0; k < 2; k++) roff[k] = 4 * min(64 * k + t, row_words - 1);
+    const bool need_un = kind != UPD_EULER;
+    const unsigned bytes = (unsigned)(g.plane * 4 * sizeof(double));
+    const i32x4 ry = buffer_rsrc(Y, bytes), rf = buffer_rsrc(F, bytes), ru = buffer_rsrc(Un, bytes);
+    const i32x4 rg = buffer_rsrc(G, bytes), ro = buffer_rsrc(out, bytes);
+    const int fb = f_lane * (int)sizeof(double);  // byte offset of the lane's face in a row
+    // byte offsets below 4 GB (march_fits), formed in unsigned arithmetic
+    const unsigned pl8 = (unsigned)(g.plane * sizeof(double)), ld8 = (unsigned)g.ld * 8u;
+    auto fetch_window = [&](int i, int buf) {
+        const unsigned o = (unsigned)i * ld8 + (unsigned)(f0 - 2) * 8u;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            lds_fetch<3>(ry, (int)(o + v * pl8), win + (buf * 4 + v) * kWinPitch, woff);
+    };
+    auto fetch_rows = [&](int i) {
+        const unsigned o = (unsigned)i * ld8 + (unsigned)(f0 + 1) * 8u;
+#pragma unroll
+        for (int v = 0; v < 4; v++) lds_fetch<2>(rf, (int)(o + v * pl8), frow + v * kRowPitch, roff);
+        if (need_un) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                lds_fetch<2>(ru, (int)(o + v * pl8), urow + v * kRowPitch, roff);
+        }
+    };
+    double fl[4] = {0.0, 0.0, 0.0, 0.0};
+    if (cell_ok) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) fl[k] = F[(size_t)k * g.plane + (size_t)(i0 - 1) * g.ld + c_lane];
+    }
+    fetch_window(i0, 0);
+    fetch_rows(i0);
+    wait_vmem();
+    for (int row = i0; row < i1; row++) {
+        // Row and lane indices pass through empty asm statements, so that the compiler
+        // forms each store address afresh instead of carrying ~20 per-lane 64-bit
+        // pointers across the loop (40 VGPRs).
+        int i = row, f = f_lane, c = c_lane;
+        asm volatile("" : "+s"(i));
+        asm volatile("" : "+v"(f), "+v"(c));
+        const int cur = (i - i0) & 1;
+        const double* w = win + cur * 4 * kWinPitch;
+        if (i + 1 < i1) fetch_window(i + 1, cur ^ 1);  // window cur ^ 1 was last read in row i - 1
+        double gf[4] = {0.0, 0.0, 0.0, 0.0};
+        bool ok = true;
+        if (face_ok) {
+            double st[6][4];
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) st[n][v] = w[v * kWinPitch + t + n];
+            ok = op.compute(st, gf);
+        }
+        wait_vmem();  // window i + 1, the F / U^n rows of i, the stores of row i - 1
+        int pending = 0;
+        const unsigned rowb = (unsigned)i * ld8;
+        if (face_ok) {
+            if (ok) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) buffer_store_f64(gf[v], rg, fb, (int)(rowb + v * pl8), 0);
+            } else {
+                op.defer((size_t)i * g.ld + f);
+                pending = 1;
+            }
+        }
+        double gu[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
+        const int pending_up = __shfl_down(pending, 1, 64);
+        double lx = 0.0, ly = 0.0;
+        if (cell_ok) {
+            double y[4], un[4], fu[4], W[4], R[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                y[v] = w[v * kWinPitch + t + 3];
+                fu[v] = frow[v * kRowPitch + t];
+                un[v] = need_un ? urow[v * kRowPitch + t] : 0.0;
+            }
+            cell_update_vals(ph, dt, kind, y, un, fl, fu, gf, gu, W, R);
+#pragma unroll
+            for (int v = 0; v < 4; v++)  // cell c = f + 1: 8 bytes past the face
+                buffer_store_f64(R[v], ro, fb + (int)sizeof(double), (int)(rowb + v * pl8), 0);
+            if (ds.slot) {  // see yflux_update_body
+                double sx, sy;
+                cell_speeds(ph.gamma, R[0], R[1], R[2], R[3], sx, sy);
+                if (sx > lx) lx = sx;
+                if (sy > ly) ly = sy;
+            }
+            if (rp.cnt && !pending && !pending_up) {
+                const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+                if (!admissible(kc, W)) {
+                    const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+                    rp.list[0][k] = (i - GC) * rp.ny + (c - GC);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) fl[v] = fu[v];
+        }
+        if (ds.slot) {
+            lx = wave_max(lx);
+            ly = wave_max(ly);
+            if (t == 0) {
+                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)b.x;
+                ds.slot[2 * sw] = (unsigned long long)__double_as_longlong(lx);
+                ds.slot[2 * sw + 1] = (unsigned long long)__double_as_longlong(ly);
+            }
+        }
+        wait_lds();  // this row's reads of the window and the F / U^n rows are complete
+        if (i + 1 < i1) fetch_rows(i + 1);
+    }
+}
+
+inline dim3 yflux_march_grid(const Geom& g) {
+    const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
+    const int per = kMarchRows * kMarchWaves;
+    return dim3((nj + kStrip - 1) / kStrip, (ni + per - 1) / per);
+}
+inline dim3 yflux_march_block() { return dim3(64, kMarchWaves); }
 
 inline dim3 yflux_update_grid(const Geom& g) {
     const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;

@@ -10,7 +10,7 @@ import pytest
 
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import DEBUG_MIXED_ALL_FP64, PRECISION_MIXED
+from overheatingproblem_amd.config import DEBUG_MIXED_ALL_FP64, DEBUG_ROW_KERNEL, PRECISION_MIXED
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -125,3 +125,28 @@ def test_mixed_double_mach_reflection():
     rho, u, v, p = cases.cons_to_prim(Umx[:, GC:-GC, GC:-GC], c.gamma)
     assert np.isfinite(Umx).all() and (rho > 0).all() and (p > 0).all()
     assert rel_l1(Umx[0, GC:-GC, GC:-GC], U64[0, GC:-GC, GC:-GC]) < 1e-6
+
+
+def _row_vs_march_cases():
+    noh = cases.noh(256, positivity=True)                       # safeguard + FP64 fix-ups
+    quad = cases.four_quadrant(97, 131, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3,
+                               seed=4, characteristic=True)     # partial strip, odd rows
+    wave = cases.smooth_wave(130, 70, time_order=2, recon_order=3, scheme=3)
+    dmr = cases.dmr(192, 48)                                    # DMR sides, positivity
+    grav = cases.four_quadrant(64, 190, cases.LAX_LIU_3, source=1, gravity=-0.5,
+                               scheme=1, characteristic=False)  # gravity source term
+    return {"noh": noh, "quad": quad, "wave": wave, "dmr": dmr, "grav": grav}
+
+
+@pytest.mark.parametrize("name", ["noh", "quad", "wave", "dmr", "grav"])
+def test_marching_and_row_kernels_identical(name):
+    """The mixed form's fused y sweep in its LDS-staged marching form (the default) and
+    one row per wavefront (RPFVM_DEBUG_ROW_KERNEL, the form for arrays of 4 GB and more)
+    give the same state, bit for bit, with dt from the device reduction."""
+    cfg, U0 = _row_vs_march_cases()[name]
+    c = mixed(cfg)
+    Um, dm, fm = run(c, U0, steps=12)
+    Ur, dr, fr = run(mixed(cfg, debug_flags=DEBUG_ROW_KERNEL), U0, steps=12)
+    assert np.array_equal(dm, dr)
+    assert np.array_equal(Um, Ur, equal_nan=True)
+    assert fm == fr
