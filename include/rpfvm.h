@@ -82,10 +82,11 @@ enum {
 /* rpfvm_config.debug_flags bits. */
 enum {
     RPFVM_DEBUG_MIXED_ALL_FP64 = 1,  /* every mixed-form face through the FP64 recompute list */
-    RPFVM_DEBUG_ROW_KERNEL = 2       /* the fused y sweep (both forms) as one row per
-                                        wavefront (the form used for arrays of 4 GB and more)
-                                        instead of the LDS-staged marching form; identical
-                                        results */
+    RPFVM_DEBUG_LARGE_ARRAY_FORMS = 2 /* the kernel forms used when an array (4 planes)
+                                        reaches 4 GB, at any size: the fused y sweep one row
+                                        per wavefront instead of the LDS-staged marching form,
+                                        global instead of buffer addressing in the mixed x
+                                        sweep; identical results */
 };
 
 /* Error codes (< 0). */
@@ -121,7 +122,7 @@ typedef struct rpfvm_config {
     int32_t debug_flags;       /* 0 normally; RPFVM_DEBUG_* bits select verification-only
                                   paths (RPFVM_DEBUG_MIXED_ALL_FP64: the mixed form routes
                                   every face through its FP64 recompute list;
-                                  RPFVM_DEBUG_ROW_KERNEL: see above) */
+                                  RPFVM_DEBUG_LARGE_ARRAY_FORMS: see above) */
     int32_t bc[4];             /* left, right, bottom, top: RPFVM_BC_* */
     double dx, dy;
     double gamma, cfl, gravity;

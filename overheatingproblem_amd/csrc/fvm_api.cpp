@@ -190,7 +190,7 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ph.source = c.source ? 1 : 0;
     x->ph.positivity = c.positivity ? 1 : 0;
     x->ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;
-    x->ph.row_kernel = (c.debug_flags & RPFVM_DEBUG_ROW_KERNEL) ? 1 : 0;
+    x->ph.large_forms = (c.debug_flags & RPFVM_DEBUG_LARGE_ARRAY_FORMS) ? 1 : 0;
     x->flux = c.precision == RPFVM_PRECISION_MIXED
                   ? select_flux_mixed(c.recon_order, c.characteristic != 0, c.scheme)
                   : select_flux(c.recon_order, c.characteristic != 0, c.scheme);

@@ -9,6 +9,7 @@
 // the fused path is bit-identical to k_flux<2> + k_update.
 #pragma once
 
+#include "fvm_buffer.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 
@@ -261,8 +262,8 @@ constexpr int kFusedRows = RPFVM_FUSED_ROWS;  // grid rows per workgroup of the 
 // ------------------------------------------------------------------ marching form
 // A wavefront takes kMarchRows consecutive rows of one 63-cell strip and walks down them.
 // While it computes row i, the stencil window of row i + 1 (cells f0-2 .. f0+66, 4
-// variables) is on its way into LDS by direct global->LDS loads (global_load_lds_dword:
-// each lane moves one dword and no VGPR holds it), and the x-flux and U^n rows of i + 1
+// variables) is on its way into LDS by direct loads (buffer_load_dword ... lds: each lane
+// moves one dword and no VGPR holds it), and the x-flux and U^n rows of i + 1
 // follow as soon as row i's update has read its own.  A cell's lower x-flux row is the
 // upper row of the cell above, carried in registers.  The wait for those loads sits after
 // the face arithmetic and before the row's first store: by then they have arrived, and
@@ -287,28 +288,6 @@ constexpr int kMarchLds = 2 * 4 * kWinPitch + 2 * 4 * kRowPitch;
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void wait_lds() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-typedef __attribute__((address_space(3))) void lds_void;
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-
-// buffer_load_dword ... lds: LDS[M0 + 4 * lane] = dword at rsrc.base + voffset + soffset.
-__device__ void buffer_load_lds_dword(i32x4 rsrc, lds_void* lds, int size, int voffset, int soffset,
-                                      int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
-
-__device__ void buffer_store_f64(double v, i32x4 rsrc, int voffset, int soffset,
-                                 int aux) __asm("llvm.amdgcn.raw.buffer.store.f64");
-
-// Buffer descriptor over `bytes` bytes from `base` (raw, stride 0; gfx9 word 3); loads
-// past `bytes` return zero.
-__device__ __forceinline__ i32x4 buffer_rsrc(const void* base, unsigned bytes) {
-    const unsigned long long a = reinterpret_cast<unsigned long long>(base);
-    i32x4 r;
-    r.x = __builtin_amdgcn_readfirstlane((int)(unsigned)a);
-    r.y = __builtin_amdgcn_readfirstlane((int)(unsigned)(a >> 32));
-    r.z = __builtin_amdgcn_readfirstlane((int)bytes);
-    r.w = 0x00020000;
-    return r;
-}
-
 // NI x 64 dwords into dst in LDS: dword 64k + t of dst gets the dword at byte
 // soffset + voff[k] of the buffer, where voff[k] = 4 min(64k + t, valid dwords - 1) (every
 // lane loads; the copies past the valid range are never read).
@@ -319,9 +298,7 @@ __device__ __forceinline__ void lds_fetch(i32x4 rsrc, int soffset, double* dst, 
     for (int k = 0; k < NI; k++) buffer_load_lds_dword(rsrc, (lds_void*)(d + 64 * k), 4, voff[k], soffset, 0, 0);
 }
 
-// The marching form addresses each input array through one buffer descriptor with 32-bit
-// offsets: the host uses it only when 4 planes fit in 4 GB (march_fits).
-inline bool march_fits(const Geom& g) { return (unsigned long long)g.plane * 4 * sizeof(double) < (1ull << 32); }
+// The marching form addresses every array through a buffer descriptor (buffer_fits).
 
 template <typename FaceOp>
 __device__ __forceinline__ void yflux_update_march(const double* Un, const double* Y, double* out,
@@ -354,11 +331,11 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
 #pragma unroll
     for (int k = 0; k < 2; k++) roff[k] = 4 * min(64 * k + t, row_words - 1);
     const bool need_un = kind != UPD_EULER;
-    const unsigned bytes = (unsigned)(g.plane * 4 * sizeof(double));
+    const unsigned bytes = (unsigned)array_bytes(g);
     const i32x4 ry = buffer_rsrc(Y, bytes), rf = buffer_rsrc(F, bytes), ru = buffer_rsrc(Un, bytes);
     const i32x4 rg = buffer_rsrc(G, bytes), ro = buffer_rsrc(out, bytes);
     const int fb = f_lane * (int)sizeof(double);  // byte offset of the lane's face in a row
-    // byte offsets below 4 GB (march_fits), formed in unsigned arithmetic
+    // byte offsets below 4 GB (buffer_fits), formed in unsigned arithmetic
     const unsigned pl8 = (unsigned)(g.plane * sizeof(double)), ld8 = (unsigned)g.ld * 8u;
     auto fetch_window = [&](int i, int buf) {
         const unsigned o = (unsigned)i * ld8 + (unsigned)(f0 - 2) * 8u;

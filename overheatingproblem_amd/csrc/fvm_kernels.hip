@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "fvm_bc.hpp"
+#include "fvm_buffer.hpp"
 #include "fvm_fused.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
@@ -484,7 +485,7 @@ static void launch_yflux_update(const double* Un, const double* Y, double* out, 
                                 double* G, const Geom& g, const Phys& ph, double dt, int kind,
                                 const Repair& rp, const FaceFix&, unsigned, const DtSlots& ds,
                                 unsigned* faults, hipStream_t s) {
-    if (!ph.row_kernel && march_fits(g)) {
+    if (use_buffer_forms(g, ph)) {
         hipLaunchKernelGGL((k_yflux_march<ORDER, CHAR, SOLVER>), yflux_march_grid(g), yflux_march_block(), 0,
                            s, Un, Y, out, F, G, g, ph, dt, kind, rp, ds, faults);
         return;

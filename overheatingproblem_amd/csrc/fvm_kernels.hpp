@@ -52,7 +52,7 @@ struct Phys {
     double idx, idy;      // 1 / dx, 1 / dy (host-computed; IEEE division rounds alike)
     int source, positivity;
     int force_fix;   // mixed form: send every face to the FP64 recompute list
-    int row_kernel;  // fused y sweep one row per wavefront (not the marching form)
+    int large_forms;  // the kernel forms for arrays of 4 GB and more, at any size
 };
 
 enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4,

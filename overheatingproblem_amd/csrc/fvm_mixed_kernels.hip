@@ -8,6 +8,7 @@
 // Keeping the rare FP64 path out of the hot kernel keeps its registers and occupancy.
 #include <hip/hip_runtime.h>
 
+#include "fvm_buffer.hpp"
 #include "fvm_fused.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
@@ -30,20 +31,24 @@ __device__ __forceinline__ int face_fp64(const Phys& ph, const double st[6][4], 
     return riemann<DIR, SOLVER>(kc, UL, UR, f);
 }
 
-template <int DIR, int ORDER, bool CHAR, int SOLVER>
+// BUF: loads and stores through buffer descriptors (fvm_buffer.hpp), for arrays below 4 GB.
+template <int DIR, int ORDER, bool CHAR, int SOLVER, bool BUF>
 __global__ FLUX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
                                                     double* __restrict__ F, Geom g, Phys ph,
                                                     FaceFix fix) {
-    const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
-    const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
+    const int j0 = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64;
+    const int j = j0 + threadIdx.x;
+    const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y +
+                  (BUF ? __builtin_amdgcn_readfirstlane((int)threadIdx.y) : (int)threadIdx.y);
     if (i >= g.R - GC || j >= g.C - GC) return;
     const size_t c0 = (size_t)i * g.ld + j;
     const size_t step = DIR == 1 ? (size_t)g.ld : 1;
     double st[6][4];
+    FaceIO<BUF> io(Y, F, g, i, j0, DIR == 1 ? g.ld : 1);
 #pragma unroll
     for (int n = 0; n < 6; n++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2) * step];
+        for (int v = 0; v < 4; v++) st[n][v] = io.load(Y, v, n - 2, c0, step);
     double UL[4], UR[4], f[4];
     mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
     if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR))) {
@@ -53,7 +58,7 @@ __global__ FLUX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
     }
     mx::riemann<DIR, SOLVER>(ph.gamma, UL, UR, f);
 #pragma unroll
-    for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
+    for (int v = 0; v < 4; v++) io.store(F, v, c0, f[v]);
 }
 
 // FP64 recomputation of the listed faces; also resets the other direction's list,
@@ -178,7 +183,7 @@ static void launch_yflux_update_mixed(const double* Un, const double* Y, double*
                                       double dt, int kind, const Repair& rp, const FaceFix& fix,
                                       unsigned stamp, const DtSlots& ds, unsigned* faults,
                                       hipStream_t s) {
-    if (!ph.row_kernel && march_fits(g)) {
+    if (use_buffer_forms(g, ph)) {
         if (ds.slot)
             hipLaunchKernelGGL((k_yflux_march_mixed<ORDER, CHAR, SOLVER, true>), yflux_march_grid(g),
                                yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
@@ -204,8 +209,12 @@ static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const P
     const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
-    hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER>), grid, block, 0, s, Y, F, g, ph,
-                       fix);
+    if (use_buffer_forms(g, ph))
+        hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER, true>), grid, block, 0, s, Y, F, g,
+                           ph, fix);
+    else
+        hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER, false>), grid, block, 0, s, Y, F, g,
+                           ph, fix);
     hipLaunchKernelGGL((k_flux_fix<DIR, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, g,
                        ph, fix, faults);
 }

@@ -10,7 +10,7 @@ import pytest
 
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import DEBUG_MIXED_ALL_FP64, DEBUG_ROW_KERNEL, PRECISION_MIXED
+from overheatingproblem_amd.config import DEBUG_MIXED_ALL_FP64, DEBUG_LARGE_ARRAY_FORMS, PRECISION_MIXED
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -139,14 +139,15 @@ def _row_vs_march_cases():
 
 
 @pytest.mark.parametrize("name", ["noh", "quad", "wave", "dmr", "grav"])
-def test_marching_and_row_kernels_identical(name):
-    """The mixed form's fused y sweep in its LDS-staged marching form (the default) and
-    one row per wavefront (RPFVM_DEBUG_ROW_KERNEL, the form for arrays of 4 GB and more)
-    give the same state, bit for bit, with dt from the device reduction."""
+def test_buffer_and_large_array_forms_identical(name):
+    """The mixed form's default kernels (fused y sweep in the LDS-staged marching form, x
+    sweep with buffer addressing) and the forms used for arrays of 4 GB and more (one row
+    per wavefront, global addressing; RPFVM_DEBUG_LARGE_ARRAY_FORMS) give the same state,
+    bit for bit, with dt from the device reduction."""
     cfg, U0 = _row_vs_march_cases()[name]
     c = mixed(cfg)
     Um, dm, fm = run(c, U0, steps=12)
-    Ur, dr, fr = run(mixed(cfg, debug_flags=DEBUG_ROW_KERNEL), U0, steps=12)
+    Ur, dr, fr = run(mixed(cfg, debug_flags=DEBUG_LARGE_ARRAY_FORMS), U0, steps=12)
     assert np.array_equal(dm, dr)
     assert np.array_equal(Um, Ur, equal_nan=True)
     assert fm == fr

@@ -13,7 +13,7 @@ import pytest
 from golden_util import cfg as golden_cfg, keys_with_suffix, load, same
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import DEBUG_ROW_KERNEL
+from overheatingproblem_amd.config import DEBUG_LARGE_ARRAY_FORMS
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -119,13 +119,14 @@ def test_oracle_matrix(oracle, scheme, recon, char, order):
 
 @pytest.mark.parametrize("scheme,recon,char,order", [(2, 5, 1, 3), (3, 5, 0, 2), (1, 3, 1, 3),
                                                      (10, 2, 0, 3)])
-def test_oracle_matrix_row_kernel(oracle, scheme, recon, char, order):
-    """The fused y sweep as one row per wavefront (the form used when an array reaches
-    4 GB; RPFVM_DEBUG_ROW_KERNEL selects it at any size) against the oracle, bit-exact."""
+def test_oracle_matrix_large_array_forms(oracle, scheme, recon, char, order):
+    """The kernel forms used when an array reaches 4 GB (the fused y sweep one row per
+    wavefront; RPFVM_DEBUG_LARGE_ARRAY_FORMS selects them at any size) against the oracle,
+    bit-exact."""
     c, U0 = cases.four_quadrant(67, 45, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3,
                                 seed=scheme * 100 + recon * 10 + char, scheme=scheme,
                                 recon_order=recon, characteristic=char, time_order=order)
-    c.debug_flags = DEBUG_ROW_KERNEL
+    c.debug_flags = DEBUG_LARGE_ARRAY_FORMS
     U = U0.copy()
     ref_dts = oracle.run_steps(c, U, 3)
     with Solver(c) as s:
