@@ -101,6 +101,7 @@ struct BatchArgs {
     int nprob;
     int rows;                  // TX: rows per strip
     int max_side;              // largest R or C of any problem (ghost-fill launch shape)
+    int buf_ok;                // every problem's arrays below 4 GB: buffer-addressed kernels
 };
 
 struct BatchStageLaunch {

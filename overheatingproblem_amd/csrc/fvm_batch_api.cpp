@@ -22,6 +22,7 @@
 #include "../../include/rpfvm.h"
 #include "fvm_api_util.hpp"
 #include "fvm_batch.hpp"
+#include "fvm_buffer.hpp"
 
 using namespace fvm;
 using fvm::api::fail;
@@ -184,6 +185,7 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
 
     // problem layout in the pools
     size_t off = 0, cells = 0;
+    bool large_forms = false;  // some problem needs the global-address kernels
     b->probs.resize(count);
     for (int k = 0; k < count; k++) {
         const rpfvm_config& c = cfgs[k];
@@ -227,11 +229,13 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             bs.lx = c.lx > 0.0 ? c.lx : c.nx * c.dx;
         }
         b->args.max_side = std::max(b->args.max_side, std::max(d.g.R, d.g.C));
+        if (!buffer_fits(d.g) || (c.debug_flags & RPFVM_DEBUG_LARGE_ARRAY_FORMS)) large_forms = true;
         off = (off + 31) / 32 * 32;  // 256-byte aligned region
         d.off = off + fvm::api::kFrontPad;
         off = d.off + 4 * d.g.plane + 32;
         cells += (size_t)c.nx * c.ny;
     }
+    b->args.buf_ok = large_forms ? 0 : 1;
     const size_t pool_doubles = (off + 31) / 32 * 32;
     auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t probs_bytes = a256(sizeof(BProb) * count);

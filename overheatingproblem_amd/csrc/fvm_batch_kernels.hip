@@ -18,6 +18,8 @@
 
 #include "fvm_batch.hpp"
 #include "fvm_bc.hpp"
+#include "fvm_buffer.hpp"
+#include "fvm_fused.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 #include "fvm_mixed.hpp"
@@ -636,6 +638,154 @@ __global__ __launch_bounds__(64) void k_batch_ysweep(BatchArgs a, int stage, int
     }
 }
 
+// k_batch_ysweep with its inputs staged through LDS, as the single path's marching form
+// (yflux_update_march): for each row of a strip, the stencil window, the x-flux row and
+// the U^n row reach LDS by direct buffer loads (no VGPRs), the next row's window requested
+// before the current row's face arithmetic and waited for after it; the lower x-flux row
+// is carried in registers.  Same arithmetic and results as k_batch_ysweep.  Used when
+// every problem's arrays are below 4 GB (32-bit buffer offsets, BatchArgs::buf_ok).
+// Waves/SIMD: mixed 4 (126 VGPRs); FP64 3 (168 VGPRs, 4 spilled; unbounded it takes 179).
+#ifndef RPFVM_BATCH_FP64_WAVES
+#define RPFVM_BATCH_FP64_WAVES 3
+#endif
+template <int ORDER, bool CHAR, bool MIXED>
+__global__ __launch_bounds__(64, MIXED ? 4 : RPFVM_BATCH_FP64_WAVES) void k_batch_ysweep_lds(BatchArgs a, int stage, int kind,
+                                                            int yi, int oi) {
+    __shared__ double lds[kMarchLds];
+    double* const win = lds;                      // [2][4][kWinPitch]
+    double* const frow = lds + 2 * 4 * kWinPitch;  // [4][kRowPitch]
+    double* const urow = frow + 4 * kRowPitch;     // [4][kRowPitch]
+    const int lane = threadIdx.x;
+    const int nact = a.sched->nact, total = a.sched->total;
+    unsigned* queue = &a.sched->queue[stage];
+    const bool need_un = kind != UPD_EULER;
+    for (int trip = 0; trip <= total; trip++) {
+        const unsigned ticket = atomicAdd(queue, lane == 0 ? 1u : 0u);
+        const int item = uniform((int)ticket);
+        if (item >= total) break;
+        const int ai = uniform(owner(a, nact, item));
+        const int k = uniform(a.act_prob[ai]);
+        const int local = item - a.act_start[ai];
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const int solver = d.solver;
+        const double dt = a.stat[k].dt;
+        const int sx = local / d.strips_y, sy = local - sx * d.strips_y;
+        const int i0 = GC + sx * a.rows;
+        const int i1 = min(i0 + a.rows, g.R - GC);
+        const int f0 = GC - 1 + sy * kStripCols;
+        const int jf = f0 + lane;  // this lane's y face G[i][jf]
+        const int c = jf + 1;      // this lane's cell column
+        const bool face_ok = jf <= g.C - GC - 1;
+        const bool cell_ok = lane < kStripCols && c <= g.C - GC - 1;
+        const unsigned bytes = (unsigned)array_bytes(g);
+        const i32x4 ry = buffer_rsrc(a.pool[yi] + d.off, bytes);
+        const i32x4 ru = buffer_rsrc(a.pool[0] + d.off, bytes);
+        const i32x4 rf = buffer_rsrc(a.flux + d.off, bytes);
+        const i32x4 ro = buffer_rsrc(a.pool[oi] + d.off, bytes);
+        const unsigned pl8 = (unsigned)g.plane * 8u, ld8 = (unsigned)g.ld * 8u;
+        // window cells f0-2 .. (<= C-1); row cells f0+1 .. (<= C-GC-1)
+        const int win_words = 2 * min(kWinCells, g.C - (f0 - 2));
+        const int row_words = 2 * min(kStripCols, g.C - GC - 1 - f0);
+        int woff[3], roff[2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) woff[q] = 4 * min(64 * q + lane, win_words - 1);
+#pragma unroll
+        for (int q = 0; q < 2; q++) roff[q] = 4 * min(64 * q + lane, row_words - 1);
+        auto fetch_window = [&](int i, int buf) {
+            const unsigned o = (unsigned)i * ld8 + (unsigned)(f0 - 2) * 8u;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                lds_fetch<3>(ry, (int)(o + v * pl8), win + (buf * 4 + v) * kWinPitch, woff);
+        };
+        auto fetch_rows = [&](int i) {
+            const unsigned o = (unsigned)i * ld8 + (unsigned)(f0 + 1) * 8u;
+#pragma unroll
+            for (int v = 0; v < 4; v++) lds_fetch<2>(rf, (int)(o + v * pl8), frow + v * kRowPitch, roff);
+            if (need_un) {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    lds_fetch<2>(ru, (int)(o + v * pl8), urow + v * kRowPitch, roff);
+            }
+        };
+        auto row_phys = [&]() {
+            const BProb* dq = a.probs + k;
+            asm volatile("" : "+s"(dq));
+            return dq->ph;
+        };
+        const int cb = c * 8;  // byte offset of the lane's cell in a row
+        double fl[4] = {0.0, 0.0, 0.0, 0.0};
+        if (cell_ok) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                fl[v] = buffer_load_f64(rf, cb, (int)((unsigned)(i0 - 1) * ld8 + v * pl8), 0);
+        }
+        fetch_window(i0, 0);
+        fetch_rows(i0);
+        wait_vmem();
+        int fault = 0;
+        for (int row = i0; row < i1; row++) {
+            const int i = opaque_row(row);
+            const int cur = (i - i0) & 1;
+            const double* w = win + cur * 4 * kWinPitch;
+            if (i + 1 < i1) fetch_window(i + 1, cur ^ 1);  // window cur ^ 1 was read in row i - 1
+            double gf[4] = {0.0, 0.0, 0.0, 0.0};
+            int bad_g = 0;
+            if (face_ok) {
+                double st[6][4];
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = w[v * kWinPitch + lane + n];
+                const Phys pr = row_phys();
+                if constexpr (MIXED) {
+                    bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
+                } else {
+                    fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                }
+            }
+            wait_vmem();  // window i + 1, the F / U^n rows of i, the stores of row i - 1
+            double gu[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
+            const int bad_gu = __shfl_down(bad_g, 1, 64);
+            if (cell_ok) {
+                double fu[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) fu[v] = frow[v * kRowPitch + lane];
+                if (MIXED && (bad_g || bad_gu)) {
+                    const unsigned q = atomicAdd(&a.sched->defer[stage], 1u);
+                    a.defer_list[q] = ((unsigned long long)k << 40) |
+                                      ((unsigned long long)i << 20) | (unsigned long long)c;
+                } else {
+                    double y[4], un[4] = {0.0, 0.0, 0.0, 0.0}, r[4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        y[v] = w[v * kWinPitch + lane + 3];
+                        if (need_un) un[v] = urow[v * kRowPitch + lane];
+                    }
+                    const Phys pc = row_phys();
+                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r);
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        buffer_store_f64(r[v], ro, cb, (int)((unsigned)i * ld8 + v * pl8), 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) fl[v] = fu[v];
+            }
+            wait_lds();  // this row's reads of the window and the F / U^n rows are complete
+            if (i + 1 < i1) fetch_rows(i + 1);
+        }
+        if (!MIXED) {
+            const unsigned long long any = __ballot(fault != 0);
+            if (any && lane == 0) {
+                atomicOr(a.faults, 1u);
+                atomicOr(&a.stat[k].faults, 1u);
+            }
+        }
+    }
+}
+
 // Wave-speed maxima of every active problem's new state (the two-sweep schedule's last
 // stage): grid (blocks, problems); the same per-cell arithmetic and NaN-ignoring max as
 // k_dt_partial.
@@ -823,8 +973,12 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
                            stage, yi);
         if constexpr (MIXED)
             hipLaunchKernelGGL((k_batch_xfix<ORDER, CHAR>), dim3(256), dim3(256), 0, s, a, stage, yi);
-        hipLaunchKernelGGL((k_batch_ysweep<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s, a,
-                           stage, kind, yi, oi);
+        if (a.buf_ok)
+            hipLaunchKernelGGL((k_batch_ysweep_lds<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s,
+                               a, stage, kind, yi, oi);
+        else
+            hipLaunchKernelGGL((k_batch_ysweep<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s, a,
+                               stage, kind, yi, oi);
         if constexpr (MIXED)
             hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(1024), dim3(256), 0, s, a, stage,
                                kind, yi, oi, 0);

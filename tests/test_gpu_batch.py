@@ -14,7 +14,8 @@ from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS, Batch, Solver
 from overheatingproblem_amd.config import (BC_FIXED_VALUE as BC_FIXED, BC_INFLOW, BC_OUTFLOW,
                                            BC_PERIODICITY as BC_PERIODIC,
-                                           BC_REFLECTION as BC_REFLECT, PRECISION_MIXED)
+                                           BC_REFLECTION as BC_REFLECT, DEBUG_LARGE_ARRAY_FORMS,
+                                           PRECISION_MIXED)
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -92,6 +93,27 @@ def test_batch_launch_shapes_agree(staging, rows, waves):
                 b.set_state(k, U)
             b.advance_to(t_end, 100000)
             outs.append([interior(b.get_state(k)) for k in range(len(cfgs))])
+    for a, c in zip(*outs):
+        assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
+def test_batch_sweeps_large_array_forms_agree(precision):
+    """The two-sweep schedule's y sweep staged through LDS with buffer addressing (used
+    when every problem's arrays are below 4 GB) and its global-address form (chosen when
+    one problem carries RPFVM_DEBUG_LARGE_ARRAY_FORMS) give the same states."""
+    cfgs, states = small_batch(precision, True, seed=11)
+    t_end = 0.02
+    outs = []
+    for flag in (0, DEBUG_LARGE_ARRAY_FORMS):
+        cs = [c.copy() for c in cfgs]
+        cs[3].debug_flags = flag
+        with Batch(cs) as b:
+            b.set_tuning(staging=STAGING_SWEEPS, rows=16, waves_per_cu=16)
+            for k, U in enumerate(states):
+                b.set_state(k, U)
+            b.advance_to(t_end, 100000)
+            outs.append([interior(b.get_state(k)) for k in range(len(cs))])
     for a, c in zip(*outs):
         assert np.array_equal(a, c)
 
