@@ -451,8 +451,10 @@ __global__ __launch_bounds__(64) void k_batch_ghosts(BatchArgs a, int side, int 
 // The x faces F[r][c] (between rows r and r + 1) of every active strip: lane t owns
 // column c = j0 + t (t < 63) and marches the strip's rows, as one row of k_flux<1> per
 // step (the first strip row of a problem also computes its lower boundary face).
-// Mixed form: a face whose FP32 states are not admissible goes to xdefer_list.
-template <int ORDER, bool CHAR, bool MIXED>
+// Mixed form: a face whose FP32 states are not admissible goes to xdefer_list.  BUF: loads
+// and stores through buffer descriptors (row offsets in SGPRs, the lane's column in one
+// VGPR; every problem's arrays below 4 GB), otherwise global addresses.
+template <int ORDER, bool CHAR, bool MIXED, bool BUF>
 __global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int yi) {
     const int lane = threadIdx.x;
     const int nact = a.sched->nact, total = a.sched->total;
@@ -480,18 +482,41 @@ __global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int
             asm volatile("" : "+s"(dq));
             return dq->ph;
         };
+        i32x4 ry, rf;
+        unsigned pl8 = 0, ld8 = 0, cb0 = 0;
+        if constexpr (BUF) {
+            const unsigned bytes = (unsigned)array_bytes(g);
+            ry = buffer_rsrc(Y, bytes);
+            rf = buffer_rsrc(F, bytes);
+            pl8 = (unsigned)g.plane * 8u;
+            ld8 = (unsigned)g.ld * 8u;
+            cb0 = (unsigned)(GC + sy * kStripCols) * 8u;  // the strip's first column
+        }
+        const int lb = lane * 8;
         for (int r = i0 == GC ? GC - 1 : i0; r < i1; r++) {
             if (!col_ok) continue;
-            // strides through an opaque copy, so that the 24 stencil offsets are formed
-            // at their loads instead of hoisted out of the row loop into live registers
-            size_t ld = g.ld, pl = g.plane;
-            asm volatile("" : "+v"(ld), "+v"(pl));
-            const size_t c0 = (size_t)opaque_row(r) * ld + c;
             double st[6][4], f[4];
+            size_t c0 = 0, pl = 0;
+            unsigned rb = 0;
+            if constexpr (BUF) {
+                rb = (unsigned)opaque_row(r) * ld8 + cb0;
 #pragma unroll
-            for (int n = 0; n < 6; n++)
+                for (int n = 0; n < 6; n++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * pl + c0 + (size_t)(n - 2) * ld];
+                    for (int v = 0; v < 4; v++)
+                        st[n][v] = buffer_load_f64(ry, lb, (int)(rb + v * pl8 + (unsigned)(n - 2) * ld8), 0);
+            } else {
+                // strides through an opaque copy, so that the 24 stencil offsets are formed
+                // at their loads instead of hoisted out of the row loop into live registers
+                size_t ld = g.ld;
+                pl = g.plane;
+                asm volatile("" : "+v"(ld), "+v"(pl));
+                c0 = (size_t)opaque_row(r) * ld + c;
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * pl + c0 + (size_t)(n - 2) * ld];
+            }
             const Phys pr = row_phys();
             if constexpr (MIXED) {
                 if (!face_mx<1, ORDER, CHAR>(pr, solver, st, f)) {
@@ -503,8 +528,13 @@ __global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int
             } else {
                 fault |= face64<1, ORDER, CHAR>(make_consts(pr), pr, solver, st, f);
             }
+            if constexpr (BUF) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) F[(size_t)v * pl + c0] = f[v];
+                for (int v = 0; v < 4; v++) buffer_store_f64(f[v], rf, lb, (int)(rb + v * pl8), 0);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) F[(size_t)v * pl + c0] = f[v];
+            }
         }
         if (!MIXED) {
             const unsigned long long any = __ballot(fault != 0);
@@ -969,8 +999,12 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
         const dim3 gg((a.max_side + 63) / 64, 4, a.nprob < 65535 ? a.nprob : 65535);
         for (int side = 0; side < 4; side++)
             hipLaunchKernelGGL(k_batch_ghosts, gg, dim3(64), 0, s, a, side, yi);
-        hipLaunchKernelGGL((k_batch_xsweep<ORDER, CHAR, MIXED>), dim3(l.grid), dim3(64), 0, s, a,
-                           stage, yi);
+        if (a.buf_ok)
+            hipLaunchKernelGGL((k_batch_xsweep<ORDER, CHAR, MIXED, true>), dim3(l.grid), dim3(64), 0,
+                               s, a, stage, yi);
+        else
+            hipLaunchKernelGGL((k_batch_xsweep<ORDER, CHAR, MIXED, false>), dim3(l.grid), dim3(64), 0,
+                               s, a, stage, yi);
         if constexpr (MIXED)
             hipLaunchKernelGGL((k_batch_xfix<ORDER, CHAR>), dim3(256), dim3(256), 0, s, a, stage, yi);
         if (a.buf_ok)
