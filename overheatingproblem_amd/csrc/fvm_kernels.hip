@@ -65,6 +65,89 @@ __global__ FLUX_BOUNDS void k_flux(const double* __restrict__ Y, double* __restr
     for (int v = 0; v < 4; v++) F[(size_t)v * g.plane + c0] = f[v];
 }
 
+// The x faces in a marching form: a wavefront takes kRingRows face rows of a 64-column
+// strip and keeps the 6 cell rows of its stencil in a ring of LDS slots (row r in slot
+// r mod 6).  Per face row it reads the stencil from LDS, then requests the one new cell
+// row (direct buffer loads into the slot just read for the last time), computes the face
+// while that row arrives, and waits for it before the flux store.  Same arithmetic as
+// k_flux; used when the arrays are below 4 GB (buffer offsets).
+#ifndef RPFVM_RING_ROWS
+#define RPFVM_RING_ROWS 8
+#endif
+#ifndef RPFVM_RING_WAVES
+#define RPFVM_RING_WAVES 1
+#endif
+constexpr int kRingRows = RPFVM_RING_ROWS;    // face rows per wavefront
+constexpr int kRingWaves = RPFVM_RING_WAVES;  // wavefronts per workgroup
+constexpr int kRingLds = 6 * 4 * 64;          // doubles per wavefront (12 KB: 3 waves/SIMD)
+
+template <int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(64 * kRingWaves, 3) void k_flux_ring(const double* __restrict__ Y,
+                                                                 double* __restrict__ F, Geom g,
+                                                                 Phys ph, unsigned* __restrict__ faults) {
+    __shared__ double lds[kRingWaves * kRingLds];
+    const Tile2 b = xcd_tile();
+    const int t = threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    // face rows ia .. ib-1 (face row i lies between cell rows i and i + 1)
+    const int ia = __builtin_amdgcn_readfirstlane(GC - 1 + (b.y * kRingWaves + wv) * kRingRows);
+    const int ib = min(ia + kRingRows, g.R - GC);
+    if (ia >= ib) return;  // whole wavefront
+    const int j0 = __builtin_amdgcn_readfirstlane(GC + b.x * 64);
+    const bool ok = j0 + t <= g.C - GC - 1;
+    double* const ring = lds + wv * kRingLds;  // [6][4][64]
+    const unsigned bytes = (unsigned)array_bytes(g);
+    const i32x4 ry = buffer_rsrc(Y, bytes), rf = buffer_rsrc(F, bytes);
+    const unsigned pl8 = (unsigned)g.plane * 8u, ld8 = (unsigned)g.ld * 8u;
+    const int row_words = 2 * min(64, g.C - GC - j0);  // the strip's cells of a row
+    int roff[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) roff[k] = 4 * min(64 * k + t, row_words - 1);
+    auto fetch_row = [&](int r) {
+        const unsigned o = (unsigned)r * ld8 + (unsigned)j0 * 8u;
+        double* slot = ring + (r % 6) * 4 * 64;
+#pragma unroll
+        for (int v = 0; v < 4; v++) lds_fetch<2>(ry, (int)(o + v * pl8), slot + v * 64, roff);
+    };
+    for (int r = ia - 2; r <= ia + 3; r++) fetch_row(r);
+    wait_vmem();
+    const Consts kc = make_consts(ph);
+    int fault = 0;
+    for (int row = ia; row < ib; row++) {
+        int i = row;
+        asm volatile("" : "+s"(i));
+        double st[6][4];
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const double* slot = ring + ((i - 2 + n) % 6) * 4 * 64;
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[n][v] = slot[v * 64 + t];
+        }
+        wait_lds();  // slot (i - 2) mod 6 is read for the last time
+        if (i + 1 < ib) fetch_row(i + 4);
+        double f[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ok) {
+            double UL[4], UR[4];
+            reconstruct_face<1, ORDER, CHAR>(kc, ph.dx, st, UL, UR);
+            if (ph.positivity && !(admissible(kc, UL) && admissible(kc, UR))) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    UL[v] = st[2][v];
+                    UR[v] = st[3][v];
+                }
+            }
+            fault |= riemann<1, SOLVER>(kc, UL, UR, f);
+        }
+        wait_vmem();  // row i + 4, the stores of face row i - 1
+        if (ok) {
+            const unsigned o = (unsigned)i * ld8 + (unsigned)j0 * 8u;
+#pragma unroll
+            for (int v = 0; v < 4; v++) buffer_store_f64(f[v], rf, t * 8, (int)(o + v * pl8), 0);
+        }
+    }
+    if (__ballot(fault != 0) && t == 0) atomicOr(faults, 1u);
+}
+
 // Solver alone on given face states, as the reference's per-solver calls
 // *_Flux(dir, var, rows, cols, GC, L_state, R_state, flux) (include/scheme.h), over the
 // face ranges of k_flux.
@@ -477,6 +560,12 @@ static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& p
     const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
+    if (DIR == 1 && use_buffer_forms(g, ph)) {
+        const int rows = g.R - 2 * GC + 1, per = kRingRows * kRingWaves;
+        hipLaunchKernelGGL((k_flux_ring<ORDER, CHAR, SOLVER>), dim3((nj + 63) / 64, (rows + per - 1) / per),
+                           dim3(64, kRingWaves), 0, s, Y, F, g, ph, faults);
+        return;
+    }
     hipLaunchKernelGGL((k_flux<DIR, ORDER, CHAR, SOLVER>), grid, block, 0, s, Y, F, g, ph, faults);
 }
 
