@@ -12,6 +12,8 @@ row blocks, which the small parity grids only begin to tile.
   the sums (the first and last faces of each line see identical stencils).
 * The mixed form stays within its accuracy target of the FP64 form on the same dts:
   relative L1(rho) <= 1e-7 after 10 steps (the headline target is 1e-5 over a run).
+* At 11600 x 11600 an array (4 planes) passes 4 GB, so the step runs the forms with 64-bit
+  global addressing (one row per wavefront in the fused sweep); the symmetry still holds.
 """
 import numpy as np
 import pytest
@@ -70,3 +72,16 @@ def test_mixed_close_to_fp64_4096():
     r64, rmx = U64[0, GC:-GC, GC:-GC], Umx[0, GC:-GC, GC:-GC]
     assert np.isfinite(rmx).all()
     assert float(np.abs(rmx - r64).mean() / np.abs(r64).mean()) <= 1e-7
+
+
+def test_transpose_symmetry_above_4gb():
+    n = 11600
+    c, U0 = cases.lax_liu_3(n)
+    assert 4 * U0[0].size * 8 >= 1 << 32  # the arrays exceed the 32-bit buffer offsets
+    U, _, faults = advance(c, U0, 2)
+    del U0
+    assert faults & 1 == 0
+    I = U[:, GC:-GC, GC:-GC]
+    scale = float(np.abs(I[0]).max())
+    for a, b in ((0, 0), (1, 2), (3, 3)):
+        assert np.abs(I[a] - I[b].T).max() / scale < 1e-11, (a, b)
