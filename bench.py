@@ -41,7 +41,7 @@ STEP_BYTES_PER_CELL = 256.0
 VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 # Kernel names behind each timing class, per form (the y sweep is fused with the
 # stage update whenever the stage writes a separate output buffer).
-KERNELS = {"fp64": {"flux_x": "k_flux", "flux_y": "k_yflux_march", "update": "k_update",
+KERNELS = {"fp64": {"flux_x": "k_flux_ring", "flux_y": "k_yflux_march", "update": "k_update",
                     "dt": "k_dt_slots", "bc": "k_bc_side"},
            "mixed": {"flux_x": "k_flux_mixed", "flux_y": "k_yflux_march_mixed",
                      "update": "k_update", "dt": "k_dt_slots", "bc": "k_bc_side"}}
