@@ -276,7 +276,7 @@ constexpr int kFusedRows = RPFVM_FUSED_ROWS;  // grid rows per workgroup of the 
 #define RPFVM_MARCH_WAVES 2
 #endif
 constexpr int kMarchRows = RPFVM_MARCH_ROWS;    // rows per wavefront
-constexpr int kMarchWaves = RPFVM_MARCH_WAVES;  // wavefronts per workgroup
+constexpr int kMarchWaves = RPFVM_MARCH_WAVES;  // wavefronts (adjacent strips) per workgroup
 constexpr int kWinCells = kStrip + 6;  // stencil window of a row: cells f0-2 .. f0+66
 constexpr int kWinPitch = 96;          // doubles per variable of a window in LDS (3 x 64 dwords)
 constexpr int kRowPitch = 64;          // doubles per variable of an F or U^n row (2 x 64 dwords)
@@ -311,13 +311,17 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
     // wave-uniform values in SGPRs (xcd_tile divides in the VALU): every address below is
     // then an SGPR base plus a lane offset
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int i0 = __builtin_amdgcn_readfirstlane(GC + (b.y * kMarchWaves + wv) * kMarchRows);
+    // the wavefronts of a workgroup take adjacent strips of the same rows, so the cache
+    // lines at their common edge are fetched once for both
+    const int strip = __builtin_amdgcn_readfirstlane(b.x * kMarchWaves + wv);
+    const int i0 = __builtin_amdgcn_readfirstlane(GC + b.y * kMarchRows);
+    if (strip * kStrip >= g.C - 2 * GC) return;  // whole wavefront
     const int i1 = min(i0 + kMarchRows, g.R - GC);
     if (i0 >= i1) return;  // whole wavefront
     double* const win = lds + wv * kMarchLds;      // [2][4][kWinPitch]
     double* const frow = win + 2 * 4 * kWinPitch;  // [4][kRowPitch]
     double* const urow = frow + 4 * kRowPitch;     // [4][kRowPitch]
-    const int f0 = __builtin_amdgcn_readfirstlane(GC - 1 + b.x * kStrip);
+    const int f0 = __builtin_amdgcn_readfirstlane(GC - 1 + strip * kStrip);
     const int f_lane = f0 + t;
     const bool face_ok = f_lane <= g.C - GC - 1;
     const int c_lane = f_lane + 1;
@@ -430,7 +434,7 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
             lx = wave_max(lx);
             ly = wave_max(ly);
             if (t == 0) {
-                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)b.x;
+                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)strip;
                 ds.slot[2 * sw] = (unsigned long long)__double_as_longlong(lx);
                 ds.slot[2 * sw + 1] = (unsigned long long)__double_as_longlong(ly);
             }
@@ -442,8 +446,8 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
 
 inline dim3 yflux_march_grid(const Geom& g) {
     const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
-    const int per = kMarchRows * kMarchWaves;
-    return dim3((nj + kStrip - 1) / kStrip, (ni + per - 1) / per);
+    const int ns = (nj + kStrip - 1) / kStrip;
+    return dim3((ns + kMarchWaves - 1) / kMarchWaves, (ni + kMarchRows - 1) / kMarchRows);
 }
 inline dim3 yflux_march_block() { return dim3(64, kMarchWaves); }
 
