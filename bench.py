@@ -314,6 +314,41 @@ def noh_full(n, t_end=0.6):
     return out
 
 
+SOLVERS = ((2, "HLLC"), (1, "HLL"), (3, "Roe"), (10, "LLF"), (11, "HLLHC"), (22, "HLLCHC"),
+           (33, "RoeHC"))
+
+
+def noh_solvers(n, t_end=0.6):
+    """Noh to t_end in the FP64 form with the positivity safeguard, once per Riemann solver
+    of the reference (schemes 1, 2, 3, 10 and the heat-capacity variants 11, 22, 33, which
+    target exactly the wall heating this workload shows): the density deficit each leaves
+    at the wall, the plateau, its fault bits and its Mcell-updates/s.  At most 6 n steps
+    (the run needs about 4 n), so a solver whose state goes non-finite stops early."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.capi import Solver
+    rows = []
+    for scheme, name in SOLVERS:
+        cfg, U0 = cases.noh(n, positivity=True, scheme=scheme)
+        with Solver(cfg) as s:
+            s.set_state(U0)
+            t0 = time.perf_counter()
+            t, steps = s.advance_to(t_end, 6 * n)
+            faults = s.sync()
+            wall = time.perf_counter() - t0
+            U = s.get_state()
+        finite = bool(np.isfinite(U).all())
+        row = {"scheme": scheme, "solver": name, "steps": steps, "t": t, "faults": faults,
+               "finite": finite, "wall_s": round(wall, 2),
+               "mcell_updates_per_s": round(n * n * steps / wall / 1e6, 1)}
+        if finite:
+            row.update(noh_metrics({"fp64": U}, n, cfg.gamma, t_end)["fp64"])
+        log(f"bench: noh {n}^2 {name}: t={t:.4f} in {steps} steps, {wall:.1f} s, faults {faults}, "
+            f"finite {finite}")
+        rows.append(row)
+    return {"workload": f"noh-{n}x{n}", "t_end": t_end, "form": "fp64",
+            "positivity_safeguard": True, "solvers": rows}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,6 +361,9 @@ def main():
     ap.add_argument("--no-dmr", action="store_true")
     ap.add_argument("--noh-full", action="store_true",
                     help="run both forms to t = 0.6 and print the accuracy report instead")
+    ap.add_argument("--noh-solvers", action="store_true",
+                    help="run the FP64 form to t = 0.6 once per Riemann solver and print the "
+                         "wall-heating report instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has "
@@ -334,6 +372,9 @@ def main():
 
     if args.noh_full:
         print(json.dumps(noh_full(args.n)), flush=True)
+        return
+    if args.noh_solvers:
+        print(json.dumps(noh_solvers(args.n)), flush=True)
         return
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.config import PRECISION_MIXED

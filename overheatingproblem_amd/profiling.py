@@ -5,6 +5,7 @@
         <cal_fetch_dir> <cal_write_dir> <cal_bytes> <name> "<commands>"
     python -m overheatingproblem_amd.profiling valu    <pmc_dir> <name> "<command>"
     python -m overheatingproblem_amd.profiling accuracy <noh_full.json> <name> "<command>"
+    python -m overheatingproblem_amd.profiling solvers <noh_solvers.json> <name> "<command>"
 
 `trace` turns a `rocprofv3 --kernel-trace --stats --output-format csv` directory into a
 markdown table of per-kernel calls and average durations.  `traffic` turns two
@@ -14,7 +15,8 @@ dispatch (overheatingproblem_amd/lib/hbm_probe), and merges the figures into
 profiles/hbm_traffic.json, which bench.py reports as roofline.traffic.  `valu` records
 per-dispatch SQ instruction counters (SQ_INSTS_VALU ...) in the same file, from which
 bench.py reports the VALU issue rate of the dominant kernel.  `accuracy` writes the
-full-length Noh accuracy record from the JSON line of `bench.py --noh-full`.
+full-length Noh accuracy record from the JSON line of `bench.py --noh-full`, `solvers`
+the per-solver wall-heating record from `bench.py --noh-solvers`.
 """
 from __future__ import annotations
 
@@ -162,6 +164,31 @@ def accuracy_markdown(d: dict, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def solvers_markdown(d: dict, command: str) -> str:
+    """The per-solver Noh wall-heating record from `bench.py --noh-solvers`'s JSON line."""
+    n = d["workload"].split("-")[1]
+    lines = [f"# Noh {n} to t = {d['t_end']}: wall heating per Riemann solver (FP64 form)", "",
+             "Command (from the repository root):", "", "```", command, "```", "",
+             "Noh implosion, gamma 5/3, p0 = 1e-6, WENO5-char, SSP-RK3, CFL 0.4, positivity "
+             "safeguard on, one MI355X.  Exact solution: rho = 16 behind the shock (r < t/3); "
+             "the wall-heating deficit is 1 - rho/16 at the origin (corner cell) and over "
+             "r < 0.05.  Fault bits (RPFVM_FAULT_*): 1 = an HLL face or RoeHC face took the "
+             "reference's bad-state skip, 2 = non-finite dt, 8 = the safeguard left a cell with "
+             "rho <= 0 or p <= 0.", "",
+             "| solver | scheme | steps | t | faults | rho(corner cell) | deficit corner | "
+             "deficit r<0.05 | rho plateau 0.1<r<0.15 | Mcell-updates/s |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for r in d["solvers"]:
+        if r["finite"]:
+            m = (f"{r['rho_corner']:.4f} | {r['deficit_corner']:+.4f} | "
+                 f"{r['deficit_r_lt_0.05']:+.5f} | {r['rho_plateau_0.1_0.15']:.4f}")
+        else:
+            m = "non-finite | | |"
+        lines.append(f"| {r['solver']} | {r['scheme']} | {r['steps']} | {r['t']:.4f} | {r['faults']} "
+                     f"| {m} | {r['mcell_updates_per_s']:.0f} |")
+    return "\n".join(lines) + "\n"
+
+
 def _merge(name: str, key: str, value) -> None:
     try:
         with open(TRAFFIC_JSON) as fh:
@@ -207,6 +234,15 @@ def main(argv):
         path = os.path.join(PROFILES, f"{name}_accuracy.md")
         with open(path, "w") as fh:
             fh.write(accuracy_markdown(d, command))
+        print(path)
+        return 0
+    if len(argv) >= 4 and argv[0] == "solvers":
+        _, jpath, name, command = argv[:4]
+        with open(jpath) as fh:
+            d = json.loads(fh.read().strip().splitlines()[-1])
+        path = os.path.join(PROFILES, f"{name}_solvers.md")
+        with open(path, "w") as fh:
+            fh.write(solvers_markdown(d, command))
         print(path)
         return 0
     if len(argv) >= 4 and argv[0] == "valu":
