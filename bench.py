@@ -314,6 +314,60 @@ def noh_full(n, t_end=0.6):
     return out
 
 
+def ll3_full(n=2048, t_end=0.3):
+    """Lax-Liu configuration 3 (the 2048^2 parity configuration) to t_end in both forms:
+    FP64 with its dts recorded, mixed on its own dts and mixed replaying the FP64 dts;
+    L1 / L-inf of rho and p of each mixed run against FP64 (absolute and relative to the
+    mean |FP64|) -- the accuracy gate of the headline metric, at full length, on a flow
+    that does not amplify round-off the way Noh's origin does."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.capi import Solver
+    from overheatingproblem_amd.config import PRECISION_MIXED
+    cfg, U0 = cases.lax_liu_3(n)
+    mx = cfg.copy()
+    mx.precision = PRECISION_MIXED
+
+    def run(c, dts=None):
+        t0 = time.perf_counter()
+        used, t, steps = [], 0.0, 0
+        with Solver(c) as s:
+            s.set_state(U0)
+            while (t < t_end) if dts is None else (steps < len(dts)):
+                if dts is None:  # the driver loop, src/RP_FVM.c:179-191
+                    dt = s.get_delta_t()
+                    if t + dt >= t_end:
+                        dt = t_end - t
+                else:
+                    dt = dts[steps]
+                used.append(dt)
+                s.time_advance(dt)
+                t += dt
+                steps += 1
+            U, faults = s.get_state(), s.sync()
+        wall = time.perf_counter() - t0
+        log(f"bench: lax-liu-3 {n}^2 to t={t} in {steps} steps, {wall:.1f} s")
+        return {"U": U, "t": t, "steps": steps, "faults": faults, "wall_s": round(wall, 2),
+                "dts": used}
+
+    from overheatingproblem_amd.cases import cons_to_prim
+    r64 = run(cfg)
+    res = {"mixed": run(mx), "mixed_replay": run(mx, dts=r64["dts"])}
+    p64 = cons_to_prim(r64["U"][:, 4:-4, 4:-4], cfg.gamma)
+    out = {"workload": f"lax-liu-3-{n}x{n}", "t_end": t_end,
+           "fp64": {"steps": r64["steps"], "t": r64["t"], "faults": r64["faults"],
+                    "wall_s": r64["wall_s"]}}
+    for k, r in res.items():
+        pm = cons_to_prim(r["U"][:, 4:-4, 4:-4], cfg.gamma)
+        d = {"steps": r["steps"], "t": r["t"], "faults": r["faults"], "wall_s": r["wall_s"]}
+        for name, a, b in (("rho", pm[0], p64[0]), ("p", pm[3], p64[3])):
+            e = np.abs(a - b)
+            d[f"l1_{name}"] = float(e.mean())
+            d[f"l1_{name}_rel"] = float(e.mean() / np.abs(b).mean())
+            d[f"linf_{name}"] = float(e.max())
+        out[k] = d
+    return out
+
+
 SOLVERS = ((2, "HLLC"), (1, "HLL"), (3, "Roe"), (10, "LLF"), (11, "HLLHC"), (22, "HLLCHC"),
            (33, "RoeHC"))
 
@@ -361,6 +415,9 @@ def main():
     ap.add_argument("--no-dmr", action="store_true")
     ap.add_argument("--noh-full", action="store_true",
                     help="run both forms to t = 0.6 and print the accuracy report instead")
+    ap.add_argument("--ll3-full", action="store_true",
+                    help="run Lax-Liu 3 at 2048^2 to t = 0.3 in both forms and print the "
+                         "mixed-vs-FP64 accuracy instead")
     ap.add_argument("--noh-solvers", action="store_true",
                     help="run the FP64 form to t = 0.6 once per Riemann solver and print the "
                          "wall-heating report instead")
@@ -375,6 +432,9 @@ def main():
         return
     if args.noh_solvers:
         print(json.dumps(noh_solvers(args.n)), flush=True)
+        return
+    if args.ll3_full:
+        print(json.dumps(ll3_full()), flush=True)
         return
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.config import PRECISION_MIXED

@@ -6,6 +6,7 @@
     python -m overheatingproblem_amd.profiling valu    <pmc_dir> <name> "<command>"
     python -m overheatingproblem_amd.profiling accuracy <noh_full.json> <name> "<command>"
     python -m overheatingproblem_amd.profiling solvers <noh_solvers.json> <name> "<command>"
+    python -m overheatingproblem_amd.profiling ll3     <ll3_full.json> <name> "<command>"
 
 `trace` turns a `rocprofv3 --kernel-trace --stats --output-format csv` directory into a
 markdown table of per-kernel calls and average durations.  `traffic` turns two
@@ -164,6 +165,25 @@ def accuracy_markdown(d: dict, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def ll3_markdown(d: dict, command: str) -> str:
+    """The Lax-Liu 3 full-length accuracy record from `bench.py --ll3-full`'s JSON line."""
+    n = d["workload"].rsplit("-", 1)[1]
+    f = d["fp64"]
+    lines = [f"# Lax-Liu configuration 3, {n}, to t = {d['t_end']}: accuracy of the mixed form",
+             "", "Command (from the repository root):", "", "```", command, "```", "",
+             "Four-quadrant Riemann problem (interface x = y = 0.8), WENO5-char + HLLC, SSP-RK3, "
+             "outflow, one MI355X.  FP64: "
+             f"{f['steps']} steps to t = {f['t']:.6f}, {f['wall_s']:.2f} s, faults {f['faults']}.",
+             "", "| mixed run | steps | wall s | L1(rho) | rel | Linf(rho) | L1(p) | rel | Linf(p) |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for label, k in (("own dts", "mixed"), ("FP64 dts", "mixed_replay")):
+        r = d[k]
+        lines.append(f"| {label} | {r['steps']} | {r['wall_s']:.2f} | {r['l1_rho']:.3e} | "
+                     f"{r['l1_rho_rel']:.3e} | {r['linf_rho']:.3e} | {r['l1_p']:.3e} | "
+                     f"{r['l1_p_rel']:.3e} | {r['linf_p']:.3e} |")
+    return "\n".join(lines) + "\n"
+
+
 def solvers_markdown(d: dict, command: str) -> str:
     """The per-solver Noh wall-heating record from `bench.py --noh-solvers`'s JSON line."""
     n = d["workload"].split("-")[1]
@@ -234,6 +254,15 @@ def main(argv):
         path = os.path.join(PROFILES, f"{name}_accuracy.md")
         with open(path, "w") as fh:
             fh.write(accuracy_markdown(d, command))
+        print(path)
+        return 0
+    if len(argv) >= 4 and argv[0] == "ll3":
+        _, jpath, name, command = argv[:4]
+        with open(jpath) as fh:
+            d = json.loads(fh.read().strip().splitlines()[-1])
+        path = os.path.join(PROFILES, f"{name}_accuracy.md")
+        with open(path, "w") as fh:
+            fh.write(ll3_markdown(d, command))
         print(path)
         return 0
     if len(argv) >= 4 and argv[0] == "solvers":
