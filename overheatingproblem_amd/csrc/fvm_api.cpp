@@ -380,7 +380,7 @@ static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
     }
     if (x->rp.cnt) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
-        launch_repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
+        x->flux.repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
                       x->stream);
     }
     return ds.slot != nullptr;

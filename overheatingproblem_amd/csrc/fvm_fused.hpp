@@ -386,21 +386,21 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
             ok = op.compute(st, gf);
         }
         wait_vmem();  // window i + 1, the F / U^n rows of i, the stores of row i - 1
-        int pending = 0;
         const unsigned rowb = (unsigned)i * ld8;
-        if (face_ok) {
-            if (ok) {
-#pragma unroll
-                for (int v = 0; v < 4; v++) buffer_store_f64(gf[v], rg, fb, (int)(rowb + v * pl8), 0);
-            } else {
-                op.defer((size_t)i * g.ld + f);
-                pending = 1;
-            }
-        }
+        const int pending = face_ok && !ok ? 1 : 0;
+        if (pending) op.defer((size_t)i * g.ld + f);
         double gu[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
-        const int pending_up = __shfl_down(pending, 1, 64);
+        const int pending_up = __shfl_down(pending, 1, 64);    // face f + 1 deferred
+        const int pending_down = __shfl_up(pending, 1, 64);    // face f - 1 deferred
+        // G: every face (FaceOp::kStoreAllG), or only those the fix-up of a deferred
+        // neighbour face reads (the two cells beside face d use faces d - 1 and d + 1; the
+        // face shared by two strips is computed by both, each seeing its own neighbour)
+        if (face_ok && ok && (FaceOp::kStoreAllG || pending_up || pending_down)) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) buffer_store_f64(gf[v], rg, fb, (int)(rowb + v * pl8), 0);
+        }
         double lx = 0.0, ly = 0.0;
         if (cell_ok) {
             double y[4], un[4], fu[4], W[4], R[4];

@@ -113,9 +113,15 @@ using FusedLaunch = void (*)(const double* Un, const double* Y, double* out, con
                              double* G, const Geom& g, const Phys& ph, double dt, int kind,
                              const Repair& rp, const FaceFix& fix, unsigned stamp,
                              const DtSlots& ds, unsigned* faults, hipStream_t s);
+// The positivity safeguard's rounds after a stage (fvm_repair.hpp).
+using RepairLaunch = void (*)(const double* Un, const double* Y, double* out, double* F, double* G,
+                              const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                              unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
+
 struct FluxPair {
     FluxLaunch x, y;
     FusedLaunch yu;
+    RepairLaunch repair;
 };
 
 FluxPair select_flux(int order, bool characteristic, int solver);

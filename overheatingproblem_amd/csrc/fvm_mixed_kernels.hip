@@ -10,6 +10,7 @@
 
 #include "fvm_buffer.hpp"
 #include "fvm_fused.hpp"
+#include "fvm_repair.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 #include "fvm_mixed.hpp"
@@ -91,6 +92,9 @@ __global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
 // k_yfix_cells once k_flux_fix<2> has recomputed it.
 template <int ORDER, bool CHAR, int SOLVER>
 struct FaceOpMx {
+    // The fused sweep stores a face flux in G only where the FP64 fix-up reads it (next to
+    // a deferred face); the safeguard recomputes the others (YFacesMx).
+    static constexpr bool kStoreAllG = false;
     const Phys& ph;
     FaceFix fix;
     __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
@@ -219,11 +223,48 @@ static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const P
                        ph, fix, faults);
 }
 
+// The y-face fluxes of the safeguard's re-updated cells in the mixed form, which does not
+// store them all: face (i, f) between cells (i, f) and (i, f + 1) comes from G when the
+// safeguard replaced it (a neighbour repaired in this stage) or when the FP32 path
+// deferred it (k_flux_fix<2> wrote its FP64 flux there); otherwise it is recomputed with
+// the fused sweep's own arithmetic, which gives the value the sweep had.
+template <int ORDER, bool CHAR, int SOLVER>
+struct YFacesMx {
+    static constexpr bool kStored = false;
+    __device__ static void face(const double* Y, const double* G, const Geom& g, const Phys& ph,
+                                const Repair& rp, unsigned stamp, int i, int f, double out[4]) {
+        const int row = (i - GC) * rp.ny;
+        const bool replaced = (f >= GC && rp.rep[row + f - GC] == stamp) ||
+                              (f + 1 <= g.C - GC - 1 && rp.rep[row + f + 1 - GC] == stamp);
+        const size_t c0 = (size_t)i * g.ld + f;
+        if (!replaced) {
+            double st[6][4];
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2)];
+            FaceOpMx<ORDER, CHAR, SOLVER> op{ph, FaceFix{}};
+            if (op.compute(st, out)) return;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[v] = G[(size_t)v * g.plane + c0];
+    }
+};
+
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_repair_mx(const double* Un, const double* Y, double* out, double* F, double* G,
+                             const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                             unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
+    launch_repair_t<YFacesMx<ORDER, CHAR, SOLVER>>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds,
+                                                   faults, s);
+}
+
 template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair mpick3() {
     return FluxPair{&launch_flux_mixed<1, ORDER, CHAR, SOLVER>,
                     &launch_flux_mixed<2, ORDER, CHAR, SOLVER>,
-                    &launch_yflux_update_mixed<ORDER, CHAR, SOLVER>};
+                    &launch_yflux_update_mixed<ORDER, CHAR, SOLVER>,
+                    &launch_repair_mx<ORDER, CHAR, SOLVER>};
 }
 
 template <int ORDER, bool CHAR>
