@@ -270,7 +270,7 @@ constexpr int kFusedRows = RPFVM_FUSED_ROWS;  // grid rows per workgroup of the 
 // no store of the row is outstanding yet.  The arithmetic and its inputs are
 // yflux_update_body's, so the results are identical.
 #ifndef RPFVM_MARCH_ROWS
-#define RPFVM_MARCH_ROWS 2
+#define RPFVM_MARCH_ROWS 4
 #endif
 #ifndef RPFVM_MARCH_WAVES
 #define RPFVM_MARCH_WAVES 2
