@@ -15,6 +15,7 @@
 
 #include "../../include/rpfvm.h"
 #include "fvm_api_util.hpp"
+#include "fvm_buffer.hpp"
 #include "fvm_kernels.hpp"
 
 using namespace fvm;
@@ -380,7 +381,11 @@ static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
     }
     if (x->rp.cnt) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
-        x->flux.repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
+        // the large-array forms store every y face (the row kernel), so the repair reads G
+        // there; otherwise the form's own repair (the mixed form recomputes what it did not
+        // store, which tests compare with the stored faces)
+        const RepairLaunch repair = use_buffer_forms(x->g, x->ph) ? x->flux.repair : &launch_repair;
+        repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
                       x->stream);
     }
     return ds.slot != nullptr;

@@ -143,7 +143,9 @@ def test_buffer_and_large_array_forms_identical(name):
     """The mixed form's default kernels (fused y sweep in the LDS-staged marching form, x
     sweep with buffer addressing) and the forms used for arrays of 4 GB and more (one row
     per wavefront, global addressing; RPFVM_DEBUG_LARGE_ARRAY_FORMS) give the same state,
-    bit for bit, with dt from the device reduction."""
+    bit for bit, with dt from the device reduction.  The default sweep stores only the y
+    fluxes its fix-up reads and the safeguard recomputes the rest; the large-array sweep
+    stores every one and the safeguard reads them (Noh: repairs in the first steps)."""
     cfg, U0 = _row_vs_march_cases()[name]
     c = mixed(cfg)
     Um, dm, fm = run(c, U0, steps=12)
