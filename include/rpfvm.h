@@ -75,8 +75,7 @@ enum {
                                         (informational, not an error) */
     RPFVM_FAULT_POSITIVITY = 8,      /* the positivity safeguard left a cell with rho <= 0
                                         or p <= 0 */
-    RPFVM_FAULT_GRID_SYNC = 16       /* internal: a grid barrier of the safeguard's
-                                        cooperative kernel did not complete */
+    RPFVM_FAULT_GRID_SYNC = 16       /* reserved (not raised) */
 };
 
 /* rpfvm_config.debug_flags bits. */
