@@ -244,7 +244,7 @@ class Solver:
 
 class Batch:
     """Many independent problems advanced together (rpfvm_batch_*).  Each problem has
-    its own Config; recon_order 5, characteristic, time_order and precision are shared.
+    its own Config; recon_order, characteristic, time_order and precision are shared.
     """
 
     def __init__(self, cfgs):

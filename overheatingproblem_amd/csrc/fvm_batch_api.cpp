@@ -151,8 +151,6 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             std::snprintf(msg, sizeof msg, "%s", rpfvm_last_error());
             return fail(e, "problem %d: %s", k, msg);
         }
-        if (c.recon_order != 5)
-            return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: the batch form runs recon_order 5", k);
         if (c.positivity)
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: no positivity safeguard in a batch", k);
         if (c.scheme == RPFVM_SCHEME_HLLHC || c.scheme == RPFVM_SCHEME_HLLCHC ||
@@ -165,9 +163,11 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         if (c.nx < 2 * GC || c.ny < 2 * GC || c.nx >= (1 << 20) - 2 * GC || c.ny >= (1 << 20) - 2 * GC)
             return fail(RPFVM_ERR_INVALID, "problem %d: nx, ny must be in 8 .. 2^20 - 9", k);
         if ((c.characteristic != 0) != (cfgs[0].characteristic != 0) ||
-            c.time_order != cfgs[0].time_order || c.precision != cfgs[0].precision)
+            c.recon_order != cfgs[0].recon_order || c.time_order != cfgs[0].time_order ||
+            c.precision != cfgs[0].precision)
             return fail(RPFVM_ERR_INVALID,
-                        "problem %d: characteristic, time_order and precision must match problem 0", k);
+                        "problem %d: recon_order, characteristic, time_order and precision must "
+                        "match problem 0", k);
     }
     if (count >= (1 << 23)) return fail(RPFVM_ERR_INVALID, "too many problems");
     rpfvm_batch* b = new (std::nothrow) rpfvm_batch();
@@ -180,7 +180,7 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     b->cfg.assign(cfgs, cfgs + count);
     b->time_order = cfgs[0].time_order;
     b->mixed = cfgs[0].precision == RPFVM_PRECISION_MIXED;
-    b->kern = select_batch(5, cfgs[0].characteristic != 0, b->mixed);
+    b->kern = select_batch(cfgs[0].recon_order, cfgs[0].characteristic != 0, b->mixed);
     b->tuning = {RPFVM_STAGING_DIRECT, 32, 8, 1, 0.0};
 
     // problem layout in the pools

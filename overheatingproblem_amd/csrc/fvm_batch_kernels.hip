@@ -1033,12 +1033,22 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
                            yi, oi, last);
 }
 
+template <int ORDER>
+BatchKernels select_order(bool characteristic, bool mixed) {
+    if (mixed)
+        return {characteristic ? &stage_launch<ORDER, true, true> : &stage_launch<ORDER, false, true>};
+    return {characteristic ? &stage_launch<ORDER, true, false> : &stage_launch<ORDER, false, false>};
+}
+
 }  // namespace
 
 BatchKernels select_batch(int order, bool characteristic, bool mixed) {
-    (void)order;  // the batch form runs fifth-order WENO (checked by the API)
-    if (mixed) return {characteristic ? &stage_launch<5, true, true> : &stage_launch<5, false, true>};
-    return {characteristic ? &stage_launch<5, true, false> : &stage_launch<5, false, false>};
+    switch (order) {  // the API admits 1, 2, 3 and 5
+        case 1: return select_order<1>(characteristic, mixed);
+        case 2: return select_order<2>(characteristic, mixed);
+        case 3: return select_order<3>(characteristic, mixed);
+        default: return select_order<5>(characteristic, mixed);
+    }
 }
 
 void launch_batch_control(const BatchArgs& a, int mode, hipStream_t s) {

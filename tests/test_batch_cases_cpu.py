@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from overheatingproblem_amd.cases import cons_to_prim
-from test_gpu_batch import small_batch
+from test_gpu_batch import ORDER_SEEDS, ORDER_T_END, order_batch, small_batch
 
 SEEDS_USED = (21, 11, 3, 8, 1, 2)
 
@@ -42,6 +42,12 @@ def test_unstable_seed_is_detected(oracle):
     assert not runs_clean(oracle, cfgs[1], states[1], 0.05)
 
 
+@pytest.mark.parametrize("order", sorted(ORDER_SEEDS))
+def test_order_batch_sets_are_stable(oracle, order):
+    cfgs, states = order_batch(0, order)
+    assert all(runs_clean(oracle, c, U, ORDER_T_END) for c, U in zip(cfgs, states))
+
+
 def test_batch_rejects_what_it_does_not_run():
     """Configuration checks happen before any device work."""
     from overheatingproblem_amd.capi import Batch, RpfvmError
@@ -51,7 +57,7 @@ def test_batch_rejects_what_it_does_not_run():
         Batch(cfgs)
     cfgs, _ = small_batch(0, True)
     cfgs[2].recon_order = 3
-    with pytest.raises(RpfvmError, match="recon_order 5"):
+    with pytest.raises(RpfvmError, match="must match problem 0"):
         Batch(cfgs)
     cfgs, _ = small_batch(0, True)
     cfgs[1].time_order = 2
