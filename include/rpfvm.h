@@ -230,9 +230,10 @@ int rpfvm_kernel_times(rpfvm_ctx *ctx, double ms[RPFVM_KCLASS_COUNT], int counts
  * Batching form: many independent problems of uneven size advanced together
  * (no counterpart in the reference, whose driver runs one problem per process,
  * src/RP_FVM.c:92-372).  Each problem keeps its own configuration (grid, dx/dy,
- * gamma, CFL, Riemann solver, boundary conditions, source) and its own dt and
- * clock; recon_order (1, 2, 3 or 5), characteristic, time_order and precision must be the
- * same for every problem, and the positivity safeguard is not available.  One RK
+ * gamma, CFL, Riemann solver -- the heat-capacity ones in FP64 batches --, boundary
+ * conditions, source) and its own dt and clock; recon_order (1, 2, 3 or 5),
+ * characteristic, time_order and precision must be the same for every problem, and the
+ * positivity safeguard is not available.  One RK
  * stage of all active problems is one persistent launch driven by a work queue
  * of strips; the whole step is captured once in a HIP graph and replayed.  Each
  * problem's interior evolves bit-identically to an rpfvm_ctx with the same

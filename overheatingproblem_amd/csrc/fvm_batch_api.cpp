@@ -153,10 +153,6 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         }
         if (c.positivity)
             return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: no positivity safeguard in a batch", k);
-        if (c.scheme == RPFVM_SCHEME_HLLHC || c.scheme == RPFVM_SCHEME_HLLCHC ||
-            c.scheme == RPFVM_SCHEME_ROEHC)
-            return fail(RPFVM_ERR_UNSUPPORTED,
-                        "problem %d: the heat-capacity schemes are not batched", k);
         for (int s = 0; s < 4; s++)
             if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
                 return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);
@@ -210,7 +206,11 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         d.ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;
         d.nx = c.nx;
         d.ny = c.ny;
-        d.solver = (c.scheme == 1 || c.scheme == 2 || c.scheme == 3) ? c.scheme : RPFVM_SCHEME_LLF;
+        d.solver = (c.scheme == RPFVM_SCHEME_HLL || c.scheme == RPFVM_SCHEME_HLLC ||
+                    c.scheme == RPFVM_SCHEME_ROE || c.scheme == RPFVM_SCHEME_HLLHC ||
+                    c.scheme == RPFVM_SCHEME_HLLCHC || c.scheme == RPFVM_SCHEME_ROEHC)
+                       ? c.scheme
+                       : RPFVM_SCHEME_LLF;
         for (int s = 0; s < 4; s++) {
             d.bc[s] = c.bc[s];
             for (int v = 0; v < 4; v++)

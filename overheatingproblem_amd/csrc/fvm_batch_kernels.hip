@@ -64,11 +64,21 @@ __device__ __forceinline__ double bload(const double* P, const BProb& d, int i, 
 }
 
 // FP64 face (the single-problem k_flux arithmetic) with the solver chosen at run time.
-template <int DIR, int ORDER, bool CHAR>
+// HC adds the heat-capacity solvers (FP64 batches only: the mixed form does not run them,
+// so its FP64 fix-ups leave them out).
+template <int DIR, int ORDER, bool CHAR, bool HC = false>
 __device__ __forceinline__ int face64(const Consts& kc, const Phys& ph, int solver,
                                       const double st[6][4], double f[4]) {
     double UL[4], UR[4];
     reconstruct_face<DIR, ORDER, CHAR>(kc, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if constexpr (HC) {
+        switch (solver) {
+            case SOLVER_HLLHC: return riemann<DIR, SOLVER_HLLHC>(kc, UL, UR, f);
+            case SOLVER_HLLCHC: return riemann<DIR, SOLVER_HLLCHC>(kc, UL, UR, f);
+            case SOLVER_ROEHC: return riemann<DIR, SOLVER_ROEHC>(kc, UL, UR, f);
+            default: break;
+        }
+    }
     switch (solver) {
         case SOLVER_HLL: return riemann<DIR, SOLVER_HLL>(kc, UL, UR, f);
         case SOLVER_HLLC: return riemann<DIR, SOLVER_HLLC>(kc, UL, UR, f);
@@ -349,7 +359,7 @@ __global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int 
             if constexpr (MIXED) {
                 return face_mx<1, ORDER, CHAR>(pr, solver, st, f);
             } else {
-                fault |= face64<1, ORDER, CHAR>(make_consts(pr), pr, solver, st, f);
+                fault |= face64<1, ORDER, CHAR, !MIXED>(make_consts(pr), pr, solver, st, f);
                 return true;
             }
         };
@@ -384,7 +394,7 @@ __global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int 
                     if constexpr (MIXED) {
                         bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
                     } else {
-                        fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                        fault |= face64<2, ORDER, CHAR, !MIXED>(make_consts(pr), pr, solver, st, gf);
                     }
                 }
                 double gu[4];
@@ -526,7 +536,7 @@ __global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int
                     continue;
                 }
             } else {
-                fault |= face64<1, ORDER, CHAR>(make_consts(pr), pr, solver, st, f);
+                fault |= face64<1, ORDER, CHAR, !MIXED>(make_consts(pr), pr, solver, st, f);
             }
             if constexpr (BUF) {
 #pragma unroll
@@ -629,7 +639,7 @@ __global__ __launch_bounds__(64) void k_batch_ysweep(BatchArgs a, int stage, int
                 if constexpr (MIXED) {
                     bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
                 } else {
-                    fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                    fault |= face64<2, ORDER, CHAR, !MIXED>(make_consts(pr), pr, solver, st, gf);
                 }
             }
             double gu[4];
@@ -771,7 +781,7 @@ __global__ __launch_bounds__(64, MIXED ? 4 : RPFVM_BATCH_FP64_WAVES) void k_batc
                 if constexpr (MIXED) {
                     bad_g = face_mx<2, ORDER, CHAR>(pr, solver, st, gf) ? 0 : 1;
                 } else {
-                    fault |= face64<2, ORDER, CHAR>(make_consts(pr), pr, solver, st, gf);
+                    fault |= face64<2, ORDER, CHAR, !MIXED>(make_consts(pr), pr, solver, st, gf);
                 }
             }
             wait_vmem();  // window i + 1, the F / U^n rows of i, the stores of row i - 1

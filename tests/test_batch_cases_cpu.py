@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from overheatingproblem_amd.cases import cons_to_prim
-from test_gpu_batch import ORDER_SEEDS, ORDER_T_END, order_batch, small_batch
+from test_gpu_batch import ORDER_SEEDS, ORDER_T_END, hc_batch, order_batch, small_batch
 
 SEEDS_USED = (21, 11, 3, 8, 1, 2)
 
@@ -45,6 +45,12 @@ def test_unstable_seed_is_detected(oracle):
 @pytest.mark.parametrize("order", sorted(ORDER_SEEDS))
 def test_order_batch_sets_are_stable(oracle, order):
     cfgs, states = order_batch(0, order)
+    assert all(runs_clean(oracle, c, U, ORDER_T_END) for c, U in zip(cfgs, states))
+
+
+@pytest.mark.parametrize("characteristic", [True, False])
+def test_heat_capacity_batch_sets_are_stable(oracle, characteristic):
+    cfgs, states = hc_batch(characteristic)
     assert all(runs_clean(oracle, c, U, ORDER_T_END) for c, U in zip(cfgs, states))
 
 

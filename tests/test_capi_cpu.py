@@ -86,7 +86,7 @@ def test_create_rejects_unsupported_forms(lib, scheme, precision, message):
 
 
 @pytest.mark.parametrize("scheme", [11, 22, 33])
-def test_batch_rejects_heat_capacity_schemes(lib, scheme):
-    cfgs = [make_config(16, 16, scheme=2), make_config(16, 16, scheme=scheme)]
-    with pytest.raises(capi.RpfvmError, match="problem 1: the heat-capacity schemes"):
+def test_mixed_batch_rejects_heat_capacity_schemes(lib, scheme):
+    cfgs = [make_config(16, 16, scheme=2, precision=1), make_config(16, 16, scheme=scheme, precision=1)]
+    with pytest.raises(capi.RpfvmError, match="problem 1: the heat-capacity schemes .* FP64 form only"):
         capi.Batch(cfgs)

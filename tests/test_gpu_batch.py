@@ -74,6 +74,17 @@ def order_batch(precision, order):
     return cfgs, states
 
 
+HC_SCHEMES = (11, 22, 33, 22, 33, 11)
+
+
+def hc_batch(characteristic):
+    """small_batch with the heat-capacity solvers HLLHC/HLLCHC/RoeHC (FP64 batches)."""
+    cfgs, states = small_batch(0, characteristic)
+    for c, sch in zip(cfgs, HC_SCHEMES):
+        c.scheme = sch
+    return cfgs, states
+
+
 @pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
 @pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 @pytest.mark.parametrize("characteristic", [True, False])
@@ -102,6 +113,25 @@ def test_batch_matches_single_problems(precision, characteristic, staging):
 def test_batch_reconstruction_orders_match_single_problems(order, precision, staging):
     """Godunov, TVD and WENO3 batches: each problem bit-identical to running it alone."""
     cfgs, states = order_batch(precision, order)
+    t_end = ORDER_T_END
+    with Batch(cfgs) as b:
+        b.set_tuning(staging=staging)
+        for k, U in enumerate(states):
+            b.set_state(k, U)
+        b.advance_to(t_end, 100000)
+        t, steps, faults = b.status()
+        got = [b.get_state(k) for k in range(len(cfgs))]
+    for k, (c, U0) in enumerate(zip(cfgs, states)):
+        ref, n, f = single(c, U0, t_end)
+        assert (int(faults[k]) & 3, int(steps[k]), float(t[k])) == (f & 3, n, t_end), k
+        assert np.isfinite(interior(ref)).all(), k
+        assert np.array_equal(interior(got[k]), interior(ref)), k
+
+
+@pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS])
+@pytest.mark.parametrize("characteristic", [True, False])
+def test_batch_heat_capacity_solvers_match_single_problems(characteristic, staging):
+    cfgs, states = hc_batch(characteristic)
     t_end = ORDER_T_END
     with Batch(cfgs) as b:
         b.set_tuning(staging=staging)
