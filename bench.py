@@ -94,12 +94,44 @@ class HipEvents:
         return float(f.value)
 
 
-def host_cores():
+def host_cpus():
+    """CPUs this process may run on (its affinity mask)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        return os.cpu_count() or 1
+
+
+def host_cores():
+    """Threads of the CPU baseline: every CPU of the affinity mask, or OMP_NUM_THREADS
+    when the environment sets it lower (a shared GPU host sets it to the CPU share that
+    goes with one GPU)."""
+    n = host_cpus()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_info(cores):
+    """What the CPU figures ran on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_model": cpu_model(), "cpus_visible": host_cpus(),
+            "omp_num_threads_env": env,
+            "why": ("OMP_NUM_THREADS of the environment (the host's CPU share for this GPU)"
+                    if env and env.isdigit() and int(env) < host_cpus()
+                    else "every CPU of the affinity mask")}
 
 
 def run_gpu(cfg, U0, steps, warmup, dts=None):
@@ -165,12 +197,69 @@ def cpu_baseline(n, steps, cores):
     value = n * n * steps / secs / 1e6
     return {"value": round(value, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
             "sample": f"{steps} RK3 steps of the same Noh workload at {n}x{n} "
-                      f"(dt + 3 stages, IO excluded), {secs:.2f} s"}
+                      f"(dt + 3 stages, IO excluded), {secs:.2f} s",
+            "host": host_info(threads)}
+
+
+def c1_full(cores):
+    """The Sedov-type blast on [-1,1]^2 at 200^2 to t = 1.0 (WENO5-char + HLLC, SSP-RK3,
+    CFL 0.5, outflow): the reference's own time loop on this host's cores
+    (oracle/_ref/ref_harness, timed as src/RP_FVM.c:181-216: dt + step, IO excluded),
+    then the FP64 form on the GPU through the same loop, and whether the two final
+    states and dt sequences are bit-identical."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.capi import Solver
+    from oracle.pyoracle import harness_run
+    cfg, U0 = cases.sedov(200)
+    t_end = 1.0
+    log(f"bench: C1 sedov 200^2 to t={t_end}: reference on {cores} threads ...")
+    rdts, rt, rU, rec = harness_run(cfg, U0, t_end, threads=cores)
+    log(f"bench: reference {len(rdts)} steps in {rec['seconds']:.2f} s; GPU FP64 ...")
+    ev = HipEvents()
+    dts = []
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        stream = s.stream()
+        e0, e1 = ev.create(), ev.create()
+        t0 = time.perf_counter()
+        ev.record(e0, stream)
+        t = 0.0
+        while t < t_end:  # src/RP_FVM.c:179-191
+            dt = s.get_delta_t()
+            if t + dt > t_end:
+                dt = t_end - t
+            if t + dt >= t_end:
+                dt = t_end - t
+            dts.append(dt)
+            s.time_advance(dt)
+            t = t + dt
+        ev.record(e1, stream)
+        gpu_ms = ev.elapsed_ms(e0, e1)
+        wall = time.perf_counter() - t0
+        U = s.get_state()
+        faults = s.sync()
+    cells = cfg.nx * cfg.ny
+
+    def same(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        return bool(a.shape == b.shape and np.array_equal(np.where(a == 0, 0.0, a),
+                                                          np.where(b == 0, 0.0, b)))
+    return {"workload": "sedov-200x200 to t=1.0", "steps": len(rdts),
+            "cpu_reference": {"value": round(cells * len(rdts) / rec["seconds"] / 1e6, 4),
+                              "unit": "Mcell-updates/s", "seconds": round(rec["seconds"], 3),
+                              "cores": rec["threads"], "kind": "reference",
+                              "host": host_info(rec["threads"])},
+            "gpu_fp64": {"value": round(cells * len(dts) / wall / 1e6, 3),
+                         "unit": "Mcell-updates/s", "seconds_wall": round(wall, 3),
+                         "gpu_ms": round(gpu_ms, 2), "steps": len(dts), "faults": faults,
+                         "note": "host-driven loop: one dt read back per step (launch-bound at 200^2)"},
+            "bit_identical": {"dts": same(dts, rdts), "state": same(U, rU), "t": t == rt}}
 
 
 def run_batch(steps, warmup, precision=1):
     """The batching form on the seeded 64-problem four-quadrant workload (sizes 256..1536,
-    HLLC/HLL/Roe per problem, WENO5-char, RK3, CFL 0.5), FP32 flux + FP64 state
+    quadrant states rho ~ U[0.1,2], p ~ U[0.02,2], u, v ~ U[-1.5,1.5], HLLC/HLL/Roe per
+    problem, WENO5-char, RK3, CFL 0.5, positivity safeguard on), FP32 flux + FP64 state
     (precision 1) or FP64: autotune, then time `steps` steps of the whole batch (every
     problem is still active: the smallest needs ~400 steps to reach t = 0.25)."""
     from overheatingproblem_amd import cases
@@ -198,7 +287,9 @@ def run_batch(steps, warmup, precision=1):
     active = int((st == warmup + steps).sum())
     value = cells * steps / (ms / 1e3) / 1e6
     return {"value": round(value, 3), "unit": "Mcell-updates/s", "ms_per_step": round(ms / steps, 4),
-            "workload": "64 four-quadrant problems, n in {256..1536}, seed 20240611",
+            "workload": "64 four-quadrant problems, n in {256..1536}, seed 20240611, "
+                        "rho U[0.1,2], p U[0.02,2], u,v U[-1.5,1.5], interface (0.5,0.5), "
+                        "HLLC/HLL/Roe, t_end 0.25, positivity safeguard",
             "cells": cells, "form": "mixed" if precision else "fp64", "tuning": tuned,
             "problems_active_throughout": active,
             "problems_with_fault_bits": {str(b): int((f == b).sum()) for b in sorted(set(int(x) for x in f))}}
@@ -419,6 +510,9 @@ def main():
     ap.add_argument("--ll3-full", action="store_true",
                     help="run Lax-Liu 3 at 2048^2 to t = 0.3 in both forms and print the "
                          "mixed-vs-FP64 accuracy instead")
+    ap.add_argument("--c1-full", action="store_true",
+                    help="run the Sedov 200^2 workload to t = 1.0 with the reference on the host "
+                         "cores and the FP64 form on the GPU, and print that record instead")
     ap.add_argument("--noh-solvers", action="store_true",
                     help="run the FP64 form to t = 0.6 once per Riemann solver and print the "
                          "wall-heating report instead")
@@ -433,6 +527,9 @@ def main():
         return
     if args.noh_solvers:
         print(json.dumps(noh_solvers(args.n)), flush=True)
+        return
+    if args.c1_full:
+        print(json.dumps(c1_full(host_cores())), flush=True)
         return
     if args.ll3_full:
         print(json.dumps(ll3_full()), flush=True)
@@ -471,13 +568,29 @@ def main():
     dom_ms, dom_n = kt[dom]
     avg_ms = dom_ms / max(1, dom_n)
     kname = KERNELS["mixed"].get(dom, dom)
-    alg_bytes = kernel_io_bytes(dom, n, n, stores_g=False)  # the mixed form's kernels
+    # Algorithmic bytes of one launch: the step's 256 B per cell-update over its three RK
+    # stages (one launch of the fused y sweep completes one stage of every cell).
+    alg_bytes = STEP_BYTES_PER_CELL / 3.0 * cells
     achieved = alg_bytes / (avg_ms / 1e3) / 1e9
+    io_bytes = kernel_io_bytes(dom, n, n, stores_g=False)  # what the mixed kernel must move
+    io_gbs = io_bytes / (avg_ms / 1e3) / 1e9
     step_gbs = STEP_BYTES_PER_CELL * cells / (ms_per_step / 1e3) / 1e9
     from overheatingproblem_amd.profiling import load_traffic
     prof = load_traffic(f"noh{n}")
     tr = prof.get("kernels", {}).get(kname)
     va = prof.get("valu", {}).get(kname)
+    # measured HBM bytes of one whole stage of the mixed form (every kernel a stage
+    # launches, per launch) against the stage's algorithmic bytes
+    stage_tr = None
+    kt_all = prof.get("kernels", {})
+    if all(k in kt_all for k in ("k_flux_mixed", "k_yflux_march_mixed")):
+        per_stage = kt_all["k_flux_mixed"]["traffic_bytes"] + kt_all["k_yflux_march_mixed"]["traffic_bytes"]
+        for small in ("k_bc_side", "k_flux_fix", "k_yfix_cells", "k_repair_block"):
+            if small in kt_all:  # per stage: 4 ghost sides, 2 fix-ups, 1 repair
+                reps = {"k_bc_side": 4, "k_flux_fix": 2}.get(small, 1)
+                per_stage += reps * kt_all[small]["traffic_bytes"]
+        stage_tr = {"bytes": round(per_stage), "vs_alg": round(per_stage / alg_bytes, 3),
+                    "bytes_per_cell_update": round(3.0 * per_stage / cells, 1)}
     valu = None
     if va:
         rate = va["valu_wave_instr_per_dispatch"] / (avg_ms / 1e3)
@@ -511,8 +624,15 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": round(tr["traffic_bytes"]) if tr else None,
                      "traffic_unit": "B per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic_vs_alg": round(tr["traffic_bytes"] / alg_bytes, 3) if tr else None,
                      "avg_launch_ms": round(avg_ms, 4), "launches": dom_n,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "alg_basis": "256 B per cell-update / 3 RK stages, one stage per launch",
+                     "stage_traffic": stage_tr,
+                     "kernel_io": {"bytes_per_launch": io_bytes, "achieved": round(io_gbs, 2),
+                                   "frac": round(io_gbs / HBM_PEAK_GBS, 5),
+                                   "basis": "the launch's own reads and writes: stage input, "
+                                            "x-face fluxes, U^n in 2 of 3 stages, output"}},
         "valu_roofline": valu,
         "step_hbm": {"achieved_gbs": round(step_gbs, 2), "frac": round(step_gbs / HBM_PEAK_GBS, 5),
                      "bytes_per_cell_update": STEP_BYTES_PER_CELL},

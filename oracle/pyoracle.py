@@ -170,3 +170,30 @@ def harness_job(mode, cfg=None, **extra):
     for k, v in extra.items():
         lines.append(f"{k} {v!r}" if isinstance(v, float) else f"{k} {v}")
     return "\n".join(lines) + "\n"
+
+
+def harness_run(cfg, U, t_end, *, max_steps=0, threads=None):
+    """The reference's own time loop (src/RP_FVM.c:179-191, next output time = Tmax =
+    t_end) through oracle/_ref/ref_harness mode "run".  Returns (dts, t, U, record): the
+    dt of every step, the final time, the final state (whole array, ghosts included) and
+    the harness's timing record (seconds of dt + step, IO excluded; thread count)."""
+    import json
+    import tempfile
+    if not os.path.exists(REF_HARNESS):
+        raise RuntimeError(f"reference harness not built: {REF_HARNESS} (make -C oracle ref)")
+    env = dict(os.environ)
+    if threads:
+        env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="true")
+    U = np.ascontiguousarray(U, dtype="<f8")
+    with tempfile.TemporaryDirectory() as td:
+        job = os.path.join(td, "job.txt")
+        with open(job, "w") as f:
+            f.write(harness_job("run", cfg, t_end=float(t_end), steps=int(max_steps)))
+        U.tofile(os.path.join(td, "in.bin"))
+        r = subprocess.run([REF_HARNESS, job, os.path.join(td, "in.bin"),
+                            os.path.join(td, "out.bin")], capture_output=True, text=True,
+                           env=env, check=True)
+        raw = np.fromfile(os.path.join(td, "out.bin"), dtype="<f8")
+    n, t = int(raw[0]), float(raw[1])
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    return raw[2:2 + n].copy(), t, raw[2 + n:].reshape(U.shape).copy(), rec

@@ -36,6 +36,10 @@
  *   dt        in: U; out: 1 double (Get_Delta_T_2D)
  *   bench     in: U; runs "steps" steps like "step", prints timing to stdout,
  *             out: dt[steps], U
+ *   run       in: U; the reference driver's time loop to "t_end" (src/RP_FVM.c:179-191
+ *             with next_output_time = Tmax = t_end, at most "steps" steps when
+ *             steps > 0), timing printed to stdout as in "bench" (dt + step, IO
+ *             excluded, src/RP_FVM.c:181-216); out: n, t, dt[n], U
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -59,7 +63,7 @@ typedef struct {
     int nx, ny, steps, time_order, scheme, recon, characteriz, source;
     int bc[4];
     int n; /* weno: number of stencils */
-    double gamma, cfl, dx, dy, gravity, fixed_dt, time;
+    double gamma, cfl, dx, dy, gravity, fixed_dt, time, t_end;
     double inflow[4];
     double fixed[4][4];
 } job_t;
@@ -103,6 +107,7 @@ static void read_job(const char *path, job_t *j) {
         else if (!strcmp(key, "gravity")) j->gravity = strtod(val, NULL);
         else if (!strcmp(key, "fixed_dt")) j->fixed_dt = strtod(val, NULL);
         else if (!strcmp(key, "time")) j->time = strtod(val, NULL);
+        else if (!strcmp(key, "t_end")) j->t_end = strtod(val, NULL);
         else if (!strncmp(key, "inflow", 6)) j->inflow[atoi(key + 6)] = strtod(val, NULL);
         else if (!strncmp(key, "fixed", 5)) {
             int side = key[5] - '0', comp = key[6] - '0';
@@ -221,6 +226,31 @@ int main(int argc, char **argv) {
             printf("{\"seconds\": %.9f, \"steps\": %d, \"cells\": %lld, \"threads\": %d}\n",
                    t1 - t0, j.steps, (long long)j.nx * j.ny, omp_get_max_threads());
         write_doubles(out, dts, (size_t)j.steps);
+        write_doubles(out, ubuf, nU);
+        free(dts);
+    } else if (!strcmp(j.mode, "run")) {
+        size_t cap = 1024, n = 0;
+        double *dts = malloc(sizeof(double) * cap);
+        double Time = 0.0, Delta_T = 0.0, wall = 0.0;
+        for (Time = 0.0; Time < j.t_end && (j.steps <= 0 || (int)n < j.steps); Time = Time + Delta_T) {
+            double t0 = omp_get_wtime();
+            Delta_T = Get_Delta_T_2D(var, rows, cols, U, j.dx, j.dy);
+            if (Time + Delta_T > j.t_end) Delta_T = j.t_end - Time;
+            if (Time + Delta_T >= j.t_end) Delta_T = j.t_end - Time;
+            run_step(j.scheme, j.time_order, rows, cols, U, Delta_T, j.dx, j.dy);
+            wall += omp_get_wtime() - t0;
+            if (n == cap) {
+                cap *= 2;
+                dts = realloc(dts, sizeof(double) * cap);
+                if (!dts) die("oom");
+            }
+            dts[n++] = Delta_T;
+        }
+        printf("{\"seconds\": %.9f, \"steps\": %zu, \"t\": %.17g, \"cells\": %lld, \"threads\": %d}\n",
+               wall, n, Time, (long long)j.nx * j.ny, omp_get_max_threads());
+        double hdr[2] = {(double)n, Time};
+        write_doubles(out, hdr, 2);
+        write_doubles(out, dts, n);
         write_doubles(out, ubuf, nU);
         free(dts);
     } else if (!strcmp(j.mode, "dt")) {

@@ -15,8 +15,10 @@ raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -58,6 +60,24 @@ class RpfvmError(RuntimeError):
 
 
 _lib = None
+
+# Every open Solver / Batch.  Destroyed at interpreter exit, before the HIP runtime's own
+# teardown (C-level exit handlers run after Python's), so that no stream, event, graph or
+# device allocation of a context outlives the runtime that owns it: a context left to
+# __del__ during interpreter shutdown may find the module globals already cleared, skip
+# rpfvm_destroy, and leave its HIP objects to the runtime's exit path.
+_live = weakref.WeakSet()
+
+
+def _close_all() -> None:
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_all)
 
 
 def lib():
@@ -131,12 +151,14 @@ class Solver:
         self.cfg = cfg.copy()
         self._ctx = ctypes.c_void_p()
         _check(lib().rpfvm_create(ctypes.byref(self.cfg), ctypes.byref(self._ctx)))
+        _live.add(self)
 
     # context management ------------------------------------------------------
     def close(self):
         if self._ctx:
-            lib().rpfvm_destroy(self._ctx)
-            self._ctx = ctypes.c_void_p()
+            ctx, self._ctx = self._ctx, ctypes.c_void_p()
+            lib().rpfvm_destroy(ctx)
+        _live.discard(self)
 
     def __enter__(self):
         return self
@@ -253,11 +275,13 @@ class Batch:
         arr = (Config * len(self.cfgs))(*self.cfgs)
         self._h = ctypes.c_void_p()
         _check(L.rpfvm_batch_create(len(self.cfgs), arr, ctypes.byref(self._h)))
+        _live.add(self)
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().rpfvm_batch_destroy(self._h)
-            self._h = None
+            h, self._h = self._h, None
+            lib().rpfvm_batch_destroy(h)
+        _live.discard(self)
 
     def __enter__(self):
         return self

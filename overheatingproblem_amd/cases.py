@@ -233,35 +233,41 @@ def random_smooth_state(nx, ny, seed, *, gamma=1.4, amp=0.3, vel=1.0, modes=3):
     return prim_to_cons(rho, u, v, p, gamma)
 
 
-def batch_problems(count=64, seed=20240611, sizes=range(256, 1537, 64)):
+# Quadrant-state ranges of the seeded batch workload: rho ~ U[0.1, 2], p ~ U[0.02, 2],
+# u, v ~ U[-1.5, 1.5].
+BATCH_RANGES = {"rho": (0.1, 2.0), "p": (0.02, 2.0), "uv": (-1.5, 1.5)}
+
+
+def batch_problems(count=64, seed=20240611, sizes=range(256, 1537, 64), ranges=BATCH_RANGES):
     """The uneven batch of seeded four-quadrant problems: n uniform over ``sizes``,
-    quadrant states rho ~ U[0.3,2], p ~ U[0.2,2], u, v ~ U[-1,1], interface at
-    (0.5, 0.5), gamma 1.4, solver uniform over {HLLC, HLL, Roe}.  (Wider ranges reach
-    near-vacuum rarefactions in which the reference's HLL, whose wave speeds are not
-    positivity bounded, meets negative states within a few steps.)  Returns a list of
-    (n, states, scheme)."""
+    quadrant states drawn from ``ranges`` (rho, u, v, p per quadrant NE, NW, SW, SE),
+    interface at (0.5, 0.5), gamma 1.4, solver uniform over {HLLC, HLL, Roe}.  Returns a
+    list of (n, states, scheme).  These ranges reach near-vacuum rarefactions, where the
+    reference's HLL (wave speeds not positivity bounded) meets negative states within a
+    few dozen steps: :func:`batch_case` turns the positivity safeguard on."""
     rng = np.random.default_rng(seed)
     sizes = list(sizes)
+    (r0, r1), (p0, p1), (v0, v1) = ranges["rho"], ranges["p"], ranges["uv"]
     out = []
     for _ in range(count):
         n = int(sizes[rng.integers(0, len(sizes))])
         states = {}
         for key in ("NE", "NW", "SW", "SE"):
-            states[key] = (float(rng.uniform(0.3, 2.0)), float(rng.uniform(-1.0, 1.0)),
-                           float(rng.uniform(-1.0, 1.0)), float(rng.uniform(0.2, 2.0)))
+            states[key] = (float(rng.uniform(r0, r1)), float(rng.uniform(v0, v1)),
+                           float(rng.uniform(v0, v1)), float(rng.uniform(p0, p1)))
         scheme = (2, 1, 3)[int(rng.integers(0, 3))]
         out.append((n, states, scheme))
     return out
 
 
 def batch_case(count=64, seed=20240611, sizes=range(256, 1537, 64), *, characteristic=True,
-               cfl=0.5, **cfg_kw):
-    """Configs and initial states of :func:`batch_problems` (outflow on every side):
-    returns (cfgs, states)."""
+               cfl=0.5, positivity=True, ranges=BATCH_RANGES, **cfg_kw):
+    """Configs and initial states of :func:`batch_problems` (outflow on every side, the
+    positivity safeguard on by default): returns (cfgs, states)."""
     cfgs, states = [], []
-    for n, quad, scheme in batch_problems(count, seed, sizes):
+    for n, quad, scheme in batch_problems(count, seed, sizes, ranges):
         c, U = four_quadrant(n, n, quad, scheme=scheme, characteristic=characteristic, cfl=cfl,
-                             **cfg_kw)
+                             positivity=positivity, **cfg_kw)
         cfgs.append(c)
         states.append(U)
     return cfgs, states

@@ -58,6 +58,7 @@ struct BProb {
     int bc[4];            // L, R, B, T
     double bcv[4][4];     // inflow / fixed state per side
     size_t off;           // offset in doubles of the problem's planes in each pool buffer
+    size_t cbase;         // first entry of the problem's interior cells in the repair arrays
     int strips_y;         // ceil(ny / 63)
     int pad;
     BcSide side[4];       // ghost fill of the two-sweep schedule (as the single path's)
@@ -71,7 +72,9 @@ struct BStat {
     int steps;                     // steps taken
     int active;                    // taking part in the step in flight
     unsigned faults;               // RPFVM_FAULT_* bits
-    int pad;
+    int mirror;                    // pools swapped every step (safeguard): bit 0, both
+                                   // pool[0] and pool[3] hold the state; bit 1, copy it
+                                   // across in the step in flight (the problem sits out)
 };
 
 // Schedule of the step in flight, written by k_batch_control.
@@ -83,8 +86,19 @@ struct BSched {
     int nact;             // active problems
     int total;            // active strips
     int active_left;      // problems with t < t_end after this step's dt
-    int pad;
+    unsigned stamp;       // positivity safeguard: stamp of stage s is stamp + s (never 0)
     double t_end;         // stop time (set by the host)
+};
+
+// Lists of the positivity safeguard over the whole batch (all null when it is off).
+// Entries are (problem << 40) | c with c = (i - GC) * ny + (j - GC) the problem's
+// interior cell; rep and tch are indexed by cbase + c.
+struct BRepair {
+    unsigned* cnt;                  // [0], [1]: list lengths, [2]: touched length, [3]: repairs
+    unsigned long long* list[2];    // cells to repair in the current / next round
+    unsigned long long* touched;    // cells to update again in the current round
+    unsigned* rep;                  // == stage stamp: repaired in this stage
+    unsigned* tch;                  // == round stamp: already in the touched list
 };
 
 struct BatchArgs {
@@ -93,7 +107,9 @@ struct BatchArgs {
     BSched* sched;
     int* act_prob;             // [nact] problem index of each active problem
     int* act_start;            // [nact + 1] first strip of each active problem
-    double* pool[3];           // U, U1, U2 of every problem (problem k at + probs[k].off)
+    double* pool[4];           // U, U1, U2 of every problem (problem k at + probs[k].off),
+                               // and with the safeguard U3, the last stage's output (the
+                               // host swaps pool[0] and pool[3] after every step)
     double* flux;              // x-face fluxes of the two-sweep schedule (same layout)
     unsigned long long* defer_list;  // (problem << 40) | (i << 20) | j of deferred cells
     unsigned long long* xdefer_list; // (problem << 40) | (i << 20) | j of deferred x faces
@@ -102,6 +118,9 @@ struct BatchArgs {
     int rows;                  // TX: rows per strip
     int max_side;              // largest R or C of any problem (ghost-fill launch shape)
     int buf_ok;                // every problem's arrays below 4 GB: buffer-addressed kernels
+    int positivity;            // the positivity safeguard (two-sweep schedule only)
+    int swap_pools;            // the last stage writes pool[3] and the host swaps it in
+    BRepair rep;
 };
 
 struct BatchStageLaunch {
@@ -121,9 +140,12 @@ BatchKernels select_batch(int order, bool characteristic, bool mixed);
 void launch_batch_control(const BatchArgs& a, int mode, hipStream_t s);
 void launch_batch_copy_back(const BatchArgs& a, hipStream_t s);
 void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s);
-// Two-sweep schedule: wave-speed maxima of every active problem's state (after RK1's
-// copy back; the other orders launch it from their last stage).
-void launch_batch_speeds_all(const BatchArgs& a, hipStream_t s);
+// Two-sweep schedule: wave-speed maxima of every active problem's state in pool[pi]
+// (after RK1's copy back; the other orders launch it from their last stage).
+void launch_batch_speeds_all(const BatchArgs& a, int pi, hipStream_t s);
 void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s);
+// Pools swapped every step: a problem that sits out a step has its state copied from
+// pool[0] to pool[3] once (k_batch_control marks it), so the swap leaves it in place.
+void launch_batch_mirror(const BatchArgs& a, hipStream_t s);
 
 }  // namespace fvm

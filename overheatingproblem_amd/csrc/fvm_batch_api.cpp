@@ -1,16 +1,26 @@
 // fvm_batch_api.cpp -- the batching form's C ABI (include/rpfvm.h, rpfvm_batch_*).
 //
 // Device memory of a batch, allocated once:
-//   three state pools (U, U1, U2) and an x-flux pool (two-sweep schedule): every
+//   three state pools (U, U1, U2), an x-flux pool (two-sweep schedule) and, with the
+//     positivity safeguard, a fourth state pool for the last stage's output: every
 //     problem's four planes of R rows x ld doubles, problem k at offset probs[k].off
 //     (interior column 128-byte aligned)
 //   probs[count] (BProb), stat[count] (BStat), one BSched, the active-problem tables,
 //   the deferred-cell and deferred-x-face lists of the mixed form (one entry per interior
-//   cell / x face at most)
-// A step is: k_batch_control, then per RK stage one persistent k_batch_stage launch
-// (+ k_batch_fix in the mixed form), + k_batch_copy_back for RK1.  It contains no
-// memset and no host synchronisation, so it is captured once into a hipGraph
-// (re-captured when the tuning changes) and replayed by rpfvm_batch_step.
+//   cell / x face at most), the safeguard's repair lists and stamps
+// A step is k_batch_control, then per RK stage the launches of the schedule the tuning
+// selects (fvm_batch_kernels.hip, stage_launch):
+//   two-sweep (the tuner's choice on the seeded workloads, the only one with the
+//     safeguard): ghost fill of the stage input, persistent x sweep into the flux pool
+//     (+ its FP64 fix-up in the mixed form), persistent y sweep fused with the stage
+//     update (+ the mixed form's cell fix-up), the safeguard's repair rounds, and after the
+//     last stage the pass that takes the next dt's wave-speed maxima;
+//   one-pass: one persistent k_batch_stage launch per stage (+ k_batch_fix in the mixed
+//     form), the maxima folded into the last stage;
+// plus k_batch_copy_back for RK1.  It contains no memset and no host synchronisation, so
+// it is captured into a hipGraph (re-captured when the tuning changes; one graph per pool
+// assignment when the safeguard alternates the state between pool[0] and pool[3]) and
+// replayed by rpfvm_batch_step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,8 +50,13 @@ struct rpfvm_batch {
     BatchArgs args{};
     BatchKernels kern{};
     rpfvm_batch_tuning tuning{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // With the safeguard the last stage writes pool[3], which becomes the state: the
+    // step's launches alternate between two pool assignments (parity), each captured in
+    // its own graph.
+    bool swap_pools = false;
+    int parity = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     BSched* host_sched = nullptr;  // pinned
     BStat* host_stat = nullptr;    // pinned
     unsigned* host_faults = nullptr;
@@ -63,31 +78,42 @@ BatchStageLaunch launch_shape(const rpfvm_batch* b) {
 // The step's launches, in stream order (also what the graph captures).
 void enqueue_step(rpfvm_batch* b) {
     launch_batch_control(b->args, 0, b->stream);
+    if (b->swap_pools) launch_batch_mirror(b->args, b->stream);
     const BatchStageLaunch l = launch_shape(b);
     for (int s = 0; s < stages_of(b->time_order); s++)
         b->kern.stage(b->args, b->time_order, s, l, b->stream);
     if (b->time_order == 1) {
         launch_batch_copy_back(b->args, b->stream);
-        if (l.staging == STAGING_SWEEPS) launch_batch_speeds_all(b->args, b->stream);
+        if (l.staging == STAGING_SWEEPS) launch_batch_speeds_all(b->args, 0, b->stream);
     }
 }
 
 void drop_graph(rpfvm_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int q = 0; q < 2; q++) {
+        if (b->exec[q]) (void)hipGraphExecDestroy(b->exec[q]);
+        if (b->graph[q]) (void)hipGraphDestroy(b->graph[q]);
+        b->exec[q] = nullptr;
+        b->graph[q] = nullptr;
+    }
 }
 
+// Graph of the step under the current pool assignment.
 int capture(rpfvm_batch* b) {
-    drop_graph(b);
+    const int q = b->parity;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(b);
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamEndCapture(b->stream, &g));
-    b->graph = g;
-    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph[q] = g;
+    HIPCHK(hipGraphInstantiate(&b->exec[q], b->graph[q], nullptr, nullptr, 0));
     return 0;
+}
+
+// After a step whose last stage wrote pool[3]: that buffer is the state now.
+void swap_state(rpfvm_batch* b) {
+    if (!b->swap_pools) return;
+    std::swap(b->args.pool[0], b->args.pool[3]);
+    b->parity ^= 1;
 }
 
 int set_t_end(rpfvm_batch* b, double t_end) {
@@ -102,22 +128,26 @@ int set_t_end(rpfvm_batch* b, double t_end) {
 
 int run_step(rpfvm_batch* b) {
     if (b->tuning.use_graph) {
-        if (!b->exec) {
+        if (!b->exec[b->parity]) {
             int e = capture(b);
             if (e) return e;
         }
-        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        HIPCHK(hipGraphLaunch(b->exec[b->parity], b->stream));
     } else {
         enqueue_step(b);
         HIPCHK(hipGetLastError());
     }
+    swap_state(b);
     return 0;
 }
 
-int check_tuning(const rpfvm_batch_tuning* t) {
+int check_tuning(const rpfvm_batch* b, const rpfvm_batch_tuning* t) {
     if (t->staging != RPFVM_STAGING_DIRECT && t->staging != RPFVM_STAGING_LDS &&
         t->staging != RPFVM_STAGING_SWEEPS)
         return fail(RPFVM_ERR_INVALID, "staging must be 0, 1 or 2");
+    if (b->args.positivity && t->staging != RPFVM_STAGING_SWEEPS)
+        return fail(RPFVM_ERR_UNSUPPORTED,
+                    "the positivity safeguard runs in the two-sweep schedule (staging 2) only");
     if (t->rows < 4 || t->rows > 1024) return fail(RPFVM_ERR_INVALID, "rows must be in 4..1024");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32)
         return fail(RPFVM_ERR_INVALID, "waves_per_cu must be in 1..32");
@@ -151,8 +181,6 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             std::snprintf(msg, sizeof msg, "%s", rpfvm_last_error());
             return fail(e, "problem %d: %s", k, msg);
         }
-        if (c.positivity)
-            return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: no positivity safeguard in a batch", k);
         for (int s = 0; s < 4; s++)
             if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
                 return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);
@@ -160,10 +188,10 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
             return fail(RPFVM_ERR_INVALID, "problem %d: nx, ny must be in 8 .. 2^20 - 9", k);
         if ((c.characteristic != 0) != (cfgs[0].characteristic != 0) ||
             c.recon_order != cfgs[0].recon_order || c.time_order != cfgs[0].time_order ||
-            c.precision != cfgs[0].precision)
+            c.precision != cfgs[0].precision || (c.positivity != 0) != (cfgs[0].positivity != 0))
             return fail(RPFVM_ERR_INVALID,
-                        "problem %d: recon_order, characteristic, time_order and precision must "
-                        "match problem 0", k);
+                        "problem %d: recon_order, characteristic, time_order, precision and "
+                        "positivity must match problem 0", k);
     }
     if (count >= (1 << 23)) return fail(RPFVM_ERR_INVALID, "too many problems");
     rpfvm_batch* b = new (std::nothrow) rpfvm_batch();
@@ -177,7 +205,11 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     b->time_order = cfgs[0].time_order;
     b->mixed = cfgs[0].precision == RPFVM_PRECISION_MIXED;
     b->kern = select_batch(cfgs[0].recon_order, cfgs[0].characteristic != 0, b->mixed);
-    b->tuning = {RPFVM_STAGING_DIRECT, 32, 8, 1, 0.0};
+    const bool pos = cfgs[0].positivity != 0;
+    b->args.positivity = pos ? 1 : 0;
+    b->swap_pools = pos && b->time_order > 1;
+    b->args.swap_pools = b->swap_pools ? 1 : 0;
+    b->tuning = {pos ? RPFVM_STAGING_SWEEPS : RPFVM_STAGING_DIRECT, 32, 8, 1, 0.0};
 
     // problem layout in the pools
     size_t off = 0, cells = 0;
@@ -202,7 +234,7 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         d.ph.idx = 1.0 / c.dx;
         d.ph.idy = 1.0 / c.dy;
         d.ph.source = c.source ? 1 : 0;
-        d.ph.positivity = 0;
+        d.ph.positivity = c.positivity ? 1 : 0;
         d.ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;
         d.nx = c.nx;
         d.ny = c.ny;
@@ -233,6 +265,7 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         off = (off + 31) / 32 * 32;  // 256-byte aligned region
         d.off = off + fvm::api::kFrontPad;
         off = d.off + 4 * d.g.plane + 32;
+        d.cbase = cells;
         cells += (size_t)c.nx * c.ny;
     }
     b->args.buf_ok = large_forms ? 0 : 1;
@@ -246,9 +279,14 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     size_t xfaces = 0;  // x faces of every problem: (nx + 1) ny
     for (int k = 0; k < count; k++) xfaces += (size_t)(cfgs[k].nx + 1) * cfgs[k].ny;
     const size_t xdefer_bytes = b->mixed ? a256(sizeof(unsigned long long) * xfaces) : 0;
-    // pools: U, U1, U2 and the two-sweep schedule's x-flux buffer
-    const size_t total = 4 * pool_doubles * sizeof(double) + probs_bytes + stat_bytes +
-                         sched_bytes + act_bytes + 256 + defer_bytes + xdefer_bytes;
+    // pools: U, U1, U2, the two-sweep schedule's x-flux buffer and, with the safeguard,
+    // U3; the safeguard's lists (3 x 8 B per cell) and stamps (2 x 4 B per cell)
+    const int npools = pos ? 5 : 4;
+    const size_t rep_bytes = pos ? 256 + 3 * a256(sizeof(unsigned long long) * cells) +
+                                       2 * a256(sizeof(unsigned) * cells)
+                                 : 0;
+    const size_t total = npools * pool_doubles * sizeof(double) + probs_bytes + stat_bytes +
+                         sched_bytes + act_bytes + 256 + defer_bytes + xdefer_bytes + rep_bytes;
 
     hipError_t he = hipGetDevice(&b->device);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(he)));
@@ -264,7 +302,8 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     char* p = static_cast<char*>(b->pool);
     for (int q = 0; q < 3; q++) b->args.pool[q] = reinterpret_cast<double*>(p) + q * pool_doubles;
     b->args.flux = reinterpret_cast<double*>(p) + 3 * pool_doubles;
-    char* r = p + 4 * pool_doubles * sizeof(double);
+    b->args.pool[3] = pos ? reinterpret_cast<double*>(p) + 4 * pool_doubles : nullptr;
+    char* r = p + npools * pool_doubles * sizeof(double);
     b->args.probs = reinterpret_cast<const BProb*>(r);
     BProb* dprobs = reinterpret_cast<BProb*>(r);
     r += probs_bytes;
@@ -280,6 +319,21 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
     b->args.defer_list = b->mixed ? reinterpret_cast<unsigned long long*>(r) : nullptr;
     r += defer_bytes;
     b->args.xdefer_list = b->mixed ? reinterpret_cast<unsigned long long*>(r) : nullptr;
+    r += xdefer_bytes;
+    if (pos) {
+        BRepair& rp = b->args.rep;
+        rp.cnt = reinterpret_cast<unsigned*>(r);
+        r += 256;
+        for (int q = 0; q < 2; q++) {
+            rp.list[q] = reinterpret_cast<unsigned long long*>(r);
+            r += a256(sizeof(unsigned long long) * cells);
+        }
+        rp.touched = reinterpret_cast<unsigned long long*>(r);
+        r += a256(sizeof(unsigned long long) * cells);
+        rp.rep = reinterpret_cast<unsigned*>(r);
+        r += a256(sizeof(unsigned) * cells);
+        rp.tch = reinterpret_cast<unsigned*>(r);
+    }
     b->args.nprob = count;
     b->args.rows = b->tuning.rows;
 
@@ -386,7 +440,7 @@ int rpfvm_batch_get_tuning(rpfvm_batch* b, rpfvm_batch_tuning* t) {
 
 int rpfvm_batch_set_tuning(rpfvm_batch* b, const rpfvm_batch_tuning* t) {
     if (!b || !t) return fail(RPFVM_ERR_INVALID, "null argument");
-    int e = check_tuning(t);
+    int e = check_tuning(b, t);
     if (e) return e;
     HIPCHK(hipStreamSynchronize(b->stream));
     b->tuning = *t;
@@ -401,14 +455,20 @@ int rpfvm_batch_tune(rpfvm_batch* b, rpfvm_batch_tuning* chosen) {
     if (!b) return fail(RPFVM_ERR_INVALID, "null batch");
     const int rows_c[] = {8, 16, 32, 64, 128};
     const int waves_c[] = {4, 8, 12, 16};
-    const int stagings[] = {RPFVM_STAGING_DIRECT, RPFVM_STAGING_LDS, RPFVM_STAGING_SWEEPS};
+    const int all_stagings[] = {RPFVM_STAGING_DIRECT, RPFVM_STAGING_LDS, RPFVM_STAGING_SWEEPS};
+    const int sweeps_only[] = {RPFVM_STAGING_SWEEPS};
+    const bool pos = b->args.positivity != 0;  // the safeguard runs in the two-sweep schedule
+    const int* const stagings = pos ? sweeps_only : all_stagings;
+    const int nstagings = pos ? 1 : 3;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    const rpfvm_batch_tuning before = b->tuning;
     rpfvm_batch_tuning best = b->tuning;
     double best_ms = 1e300;
     int err = 0;
-    for (int st : stagings) {
+    for (int si = 0; si < nstagings; si++) {
+        const int st = stagings[si];
         for (int rows : rows_c) {
             for (int w : waves_c) {
                 rpfvm_batch_tuning t{st, rows, w, b->tuning.use_graph, 0.0};
@@ -437,7 +497,13 @@ int rpfvm_batch_tune(rpfvm_batch* b, rpfvm_batch_tuning* chosen) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (err) return fail(RPFVM_ERR_HIP, "tuning launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (err) {
+        // back to the shape the batch had before the sweep, and no graph of a trial shape
+        b->tuning = before;
+        b->args.rows = before.rows;
+        drop_graph(b);
+        return fail(RPFVM_ERR_HIP, "tuning launch failed: %s", hipGetErrorString(hipGetLastError()));
+    }
     b->tuning = best;
     b->args.rows = best.rows;
     drop_graph(b);

@@ -71,6 +71,14 @@ __device__ __forceinline__ int face64(const Consts& kc, const Phys& ph, int solv
                                       const double st[6][4], double f[4]) {
     double UL[4], UR[4];
     reconstruct_face<DIR, ORDER, CHAR>(kc, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    if (ph.positivity && !(admissible(kc, UL) && admissible(kc, UR))) {
+        // the safeguard's face level: first-order states (k_flux)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            UL[v] = st[2][v];
+            UR[v] = st[3][v];
+        }
+    }
     if constexpr (HC) {
         switch (solver) {
             case SOLVER_HLLHC: return riemann<DIR, SOLVER_HLLHC>(kc, UL, UR, f);
@@ -108,11 +116,12 @@ __device__ __forceinline__ bool face_mx(const Phys& ph, int solver, const double
 // Space_Discrete_Item (include/scheme.h:3758-3777) with Source_Gravity (:3723-3751) and
 // the RK stage combine (include/time_advance.h:697, 757-777, 832-872) of one cell, with
 // the four face fluxes in registers: fl/fu the lower/upper x faces, gl/gu the y faces.
-// The expressions are cell_update's, term for term.
+// The expressions are cell_update's, term for term; W receives the forward-Euler state
+// Y + dt L that the positivity safeguard tests.
 __device__ __forceinline__ void combine(const Phys& ph, double dt, int kind, const double y[4],
                                         const double un[4], const double fl[4],
                                         const double fu[4], const double gl[4],
-                                        const double gu[4], double r[4]) {
+                                        const double gu[4], double r[4], double W[4]) {
     double S[4] = {0.0, 0.0, 0.0, 0.0};
     if (ph.source) {
         const double rho = y[0];
@@ -124,6 +133,7 @@ __device__ __forceinline__ void combine(const Phys& ph, double dt, int kind, con
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const double L = -(fu[k] - fl[k]) * idx - (gu[k] - gl[k]) * idy + S[k];
+        W[k] = y[k] + dt * L;
         switch (kind) {
             case UPD_EULER: r[k] = y[k] + dt * L; break;
             case UPD_RK2_2: r[k] = 0.5 * un[k] + 0.5 * (y[k] + dt * L); break;
@@ -179,6 +189,17 @@ __device__ __forceinline__ int owner(const BatchArgs& a, int nact, int item) {
 
 constexpr int kRingCols = 72;  // 69 columns of a strip row (+3 padding)
 
+// The positivity safeguard's first list: cell (i, c) of problem k, whose forward-Euler
+// state W is not admissible.
+__device__ __forceinline__ void repair_candidate(const BatchArgs& a, const BProb& d, const Phys& ph,
+                                                 int k, int i, int c, const double W[4]) {
+    const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
+    if (admissible(kc, W)) return;
+    const unsigned q = atomicAdd(&a.rep.cnt[0], 1u);
+    a.rep.list[0][q] = ((unsigned long long)k << 40) |
+                       (unsigned long long)((long)(i - GC) * d.ny + (c - GC));
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- control
@@ -214,6 +235,7 @@ __global__ void k_batch_control(BatchArgs a, int mode) {
                 }
             }
             s.active = act ? 1 : 0;
+            if (a.swap_pools) s.mirror = act ? 0 : ((s.mirror & 1) ? 1 : 3);
         } else {
             s.active = (s.t < t_end && !(s.faults & 2u)) ? 1 : 0;
         }
@@ -238,6 +260,7 @@ __global__ void k_batch_control(BatchArgs a, int mode) {
         sc.nact = nact;
         sc.total = total;
         sc.active_left = left;
+        sc.stamp += 4u;  // stage stamps of this step: stamp + 0, 1, 2 (the first is 4)
         for (int q = 0; q < 4; q++) {
             sc.queue[q] = 0u;
             sc.defer[q] = 0u;
@@ -413,7 +436,8 @@ __global__ __launch_bounds__(64) void k_batch_stage(BatchArgs a, int stage, int 
 #pragma unroll
                             for (int v = 0; v < 4; v++) un[v] = Un[(size_t)v * g.plane + o0];
                         }
-                        combine(ph, dt, kind, y, un, fp, fc, gf, gu, r);
+                        double W[4];
+                        combine(ph, dt, kind, y, un, fp, fc, gf, gu, r, W);
 #pragma unroll
                         for (int v = 0; v < 4; v++) out[(size_t)v * g.plane + o0] = r[v];
                         if (last) {
@@ -662,9 +686,11 @@ __global__ __launch_bounds__(64) void k_batch_ysweep(BatchArgs a, int stage, int
                         if (kind != UPD_EULER) un[v] = Un[(size_t)v * pl + o0];
                     }
                     const Phys pc = row_phys();
-                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r);
+                    double W[4];
+                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r, W);
 #pragma unroll
                     for (int v = 0; v < 4; v++) out[(size_t)v * pl + o0] = r[v];
+                    if (a.positivity) repair_candidate(a, d, pc, k, i, c, W);
                 }
             }
         }
@@ -805,10 +831,12 @@ __global__ __launch_bounds__(64, MIXED ? 4 : RPFVM_BATCH_FP64_WAVES) void k_batc
                         if (need_un) un[v] = urow[v * kRowPitch + lane];
                     }
                     const Phys pc = row_phys();
-                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r);
+                    double W[4];
+                    combine(pc, dt, kind, y, un, fl, fu, gf, gu, r, W);
 #pragma unroll
                     for (int v = 0; v < 4; v++)
                         buffer_store_f64(r[v], ro, cb, (int)((unsigned)i * ld8 + v * pl8), 0);
+                    if (a.positivity) repair_candidate(a, d, pc, k, i, c, W);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; v++) fl[v] = fu[v];
@@ -829,12 +857,12 @@ __global__ __launch_bounds__(64, MIXED ? 4 : RPFVM_BATCH_FP64_WAVES) void k_batc
 // Wave-speed maxima of every active problem's new state (the two-sweep schedule's last
 // stage): grid (blocks, problems); the same per-cell arithmetic and NaN-ignoring max as
 // k_dt_partial.
-__global__ __launch_bounds__(256) void k_batch_speeds_all(BatchArgs a) {
+__global__ __launch_bounds__(256) void k_batch_speeds_all(BatchArgs a, int pi) {
     for (int k = blockIdx.y; k < a.nprob; k += gridDim.y) {
         if (!a.stat[k].active) continue;
         const BProb& d = a.probs[k];
         const Geom g = d.g;
-        const double* U = a.pool[0] + d.off;
+        const double* U = a.pool[pi] + d.off;
         double sx = 0.0, sy = 0.0;
         const long total = (long)d.nx * d.ny;
         for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
@@ -905,9 +933,11 @@ __global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int k
             y[v] = Y[(size_t)v * g.plane + o0];
             if (kind != UPD_EULER) un[v] = Un[(size_t)v * g.plane + o0];
         }
-        combine(ph, a.stat[k].dt, kind, y, un, fl, fu, gl, gu, r);
+        double W[4];
+        combine(ph, a.stat[k].dt, kind, y, un, fl, fu, gl, gu, r, W);
 #pragma unroll
         for (int v = 0; v < 4; v++) out[(size_t)v * g.plane + o0] = r[v];
+        if (a.positivity) repair_candidate(a, d, ph, k, i, c, W);
         if (last) {
             double lx, ly;
             speeds(ph.gamma, r, lx, ly);
@@ -917,6 +947,179 @@ __global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int k
             atomicOr(a.faults, 1u);
             atomicOr(&a.stat[k].faults, 1u);
         }
+    }
+}
+
+// ------------------------------------------------------- positivity safeguard
+// The single-problem path's rounds (fvm_repair.hpp) over the batch-wide lists, in one
+// workgroup with __syncthreads() between the phases: the cells of list r are marked
+// repaired in this stage, every cell beside them (and they themselves) is touched, and
+// each touched cell is updated again; one whose forward-Euler state is still not
+// admissible goes to list r + 1, or raises RPFVM_FAULT_POSITIVITY if it was repaired
+// already.  Problems do not interact, so each problem's rounds are the single path's.
+// A face is replaced once a cell beside it is repaired (rep == stamp).  The single path
+// stores the replacement HLLE fluxes; here the cells phase evaluates them where it needs
+// them, and takes every other face from the stage's own results: x faces from the flux
+// pool, y faces recomputed with the sweep's arithmetic (the mixed form's FP32 face, or
+// its FP64 face where the sweep deferred it) -- the same values, so the same cells.
+template <int ORDER, bool CHAR, bool MIXED>
+__device__ __forceinline__ int sweep_yface(const BProb& d, const Phys& ph, const double* Y, int i,
+                                           int f, double out[4]) {
+    const Geom& g = d.g;
+    const size_t c0 = (size_t)i * g.ld + f;
+    double st[6][4];
+#pragma unroll
+    for (int n = 0; n < 6; n++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2)];
+    if constexpr (MIXED) {
+        if (face_mx<2, ORDER, CHAR>(ph, d.solver, st, out)) return 0;
+        return face64<2, ORDER, CHAR>(make_consts(ph), ph, d.solver, st, out);
+    } else {
+        return face64<2, ORDER, CHAR, true>(make_consts(ph), ph, d.solver, st, out);
+    }
+}
+
+// HLLE flux of the face between cells a and b = a + step (offsets in a plane).
+template <int DIR>
+__device__ __forceinline__ int pair_hlle(const Consts& kc, const double* Y, const Geom& g, size_t a,
+                                         size_t step, double f[4]) {
+    double l[4], r[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        l[v] = Y[(size_t)v * g.plane + a];
+        r[v] = Y[(size_t)v * g.plane + a + step];
+    }
+    return hlle<DIR>(kc, l, r, f);
+}
+
+constexpr unsigned long long kCellMask = (1ull << 40) - 1;
+
+__device__ __forceinline__ unsigned cnt_load(const unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int ORDER, bool CHAR, bool MIXED>
+__global__ __launch_bounds__(256) void k_batch_repair(BatchArgs a, int stage, int kind, int yi,
+                                                      int oi) {
+    const BRepair& rp = a.rep;
+    if (cnt_load(&rp.cnt[0]) == 0u) return;
+    const unsigned stamp = a.sched->stamp + (unsigned)stage;
+    const unsigned tid = threadIdx.x, nth = blockDim.x;
+    int r = 0;
+    for (; r < kRepairRounds; r++) {
+        const int cur = r & 1, nxt = cur ^ 1;
+        const unsigned n = cnt_load(&rp.cnt[cur]);
+        if (n == 0u) break;
+        __syncthreads();  // every thread has read n before the counters are reset
+        if (tid == 0) {
+            rp.cnt[nxt] = 0u;
+            rp.cnt[2] = 0u;
+            atomicAdd(&rp.cnt[3], n);
+        }
+        // faces: the listed cells are repaired in this stage
+        for (unsigned q = tid; q < n; q += nth) {
+            const unsigned long long e = rp.list[cur][q];
+            rp.rep[a.probs[(int)(e >> 40)].cbase + (e & kCellMask)] = stamp;
+        }
+        __syncthreads();
+        // touch: each listed cell and its four neighbours, once per round
+        const unsigned tstamp = stamp * 16u + (unsigned)r;
+        for (unsigned q = tid; q < 5u * n; q += nth) {
+            const unsigned long long e = rp.list[cur][q / 5u];
+            const int k = (int)(e >> 40);
+            const BProb& d = a.probs[k];
+            const long c = (long)(e & kCellMask);
+            const unsigned m = q % 5u;
+            int ti = (int)(c / d.ny), tj = (int)(c % d.ny);
+            ti += m == 1 ? -1 : m == 2 ? 1 : 0;
+            tj += m == 3 ? -1 : m == 4 ? 1 : 0;
+            if (ti < 0 || ti >= d.nx || tj < 0 || tj >= d.ny) continue;
+            const long cc = (long)ti * d.ny + tj;
+            if (atomicExch(&rp.tch[d.cbase + cc], tstamp) != tstamp) {
+                const unsigned w = atomicAdd(&rp.cnt[2], 1u);
+                rp.touched[w] = ((unsigned long long)k << 40) | (unsigned long long)cc;
+            }
+        }
+        __syncthreads();
+        // cells: update every touched cell again with its faces as they stand now
+        const unsigned nt = cnt_load(&rp.cnt[2]);
+        for (unsigned q = tid; q < nt; q += nth) {
+            const unsigned long long e = rp.touched[q];
+            const int k = (int)(e >> 40);
+            const BProb& d = a.probs[k];
+            const Geom g = d.g;
+            const Phys ph = d.ph;
+            const Consts kc = make_consts(ph);
+            const long c = (long)(e & kCellMask);
+            const int i = GC + (int)(c / d.ny), j = GC + (int)(c % d.ny);
+            const unsigned* rep = rp.rep + d.cbase;
+            const double* Y = a.pool[yi] + d.off;
+            const double* F = a.flux + d.off;
+            const size_t o = (size_t)i * g.ld + j;
+            const bool here = rep[c] == stamp;
+            int fault = 0;
+            double fl[4], fu[4], gl[4], gu[4];
+            if (here || (i - 1 >= GC && rep[c - d.ny] == stamp)) {
+                fault |= pair_hlle<1>(kc, Y, g, o - g.ld, g.ld, fl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) fl[v] = F[(size_t)v * g.plane + o - g.ld];
+            }
+            if (here || (i + 1 <= g.R - GC - 1 && rep[c + d.ny] == stamp)) {
+                fault |= pair_hlle<1>(kc, Y, g, o, g.ld, fu);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) fu[v] = F[(size_t)v * g.plane + o];
+            }
+            if (here || (j - 1 >= GC && rep[c - 1] == stamp))
+                fault |= pair_hlle<2>(kc, Y, g, o - 1, 1, gl);
+            else
+                fault |= sweep_yface<ORDER, CHAR, MIXED>(d, ph, Y, i, j - 1, gl);
+            if (here || (j + 1 <= g.C - GC - 1 && rep[c + 1] == stamp))
+                fault |= pair_hlle<2>(kc, Y, g, o, 1, gu);
+            else
+                fault |= sweep_yface<ORDER, CHAR, MIXED>(d, ph, Y, i, j, gu);
+            double y[4], un[4] = {0.0, 0.0, 0.0, 0.0}, res[4], W[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                y[v] = Y[(size_t)v * g.plane + o];
+                if (kind != UPD_EULER) un[v] = a.pool[0][d.off + (size_t)v * g.plane + o];
+            }
+            combine(ph, a.stat[k].dt, kind, y, un, fl, fu, gl, gu, res, W);
+            double* out = a.pool[oi] + d.off;
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[(size_t)v * g.plane + o] = res[v];
+            if (fault) {
+                atomicOr(a.faults, 1u);
+                atomicOr(&a.stat[k].faults, 1u);
+            }
+            if (!admissible(kc, W)) {
+                if (!here) {
+                    const unsigned w = atomicAdd(&rp.cnt[nxt], 1u);
+                    rp.list[nxt][w] = e;
+                } else {
+                    atomicOr(a.faults, 8u);
+                    atomicOr(&a.stat[k].faults, 8u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (r == kRepairRounds) {  // cells still listed after the last round
+        const int cur = kRepairRounds & 1;
+        const unsigned n = cnt_load(&rp.cnt[cur]);
+        for (unsigned q = tid; q < n; q += nth) {
+            const int k = (int)(rp.list[cur][q] >> 40);
+            atomicOr(a.faults, 8u);
+            atomicOr(&a.stat[k].faults, 8u);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        rp.cnt[0] = 0u;
+        rp.cnt[1] = 0u;
+        rp.cnt[2] = 0u;
     }
 }
 
@@ -938,6 +1141,24 @@ __global__ __launch_bounds__(256) void k_batch_copy_back(BatchArgs a) {
     }
 }
 
+// Pools swapped every step: the interior of each problem that sits out this step and
+// is not mirrored yet, pool[0] -> pool[3] (k_batch_control set bit 1 of its flags).
+__global__ __launch_bounds__(256) void k_batch_mirror(BatchArgs a) {
+    for (int k = blockIdx.y; k < a.nprob; k += gridDim.y) {
+        if (!(a.stat[k].mirror & 2)) continue;
+        const BProb& d = a.probs[k];
+        const Geom g = d.g;
+        const long total = (long)d.nx * d.ny;
+        for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+             t += (long)gridDim.x * blockDim.x) {
+            const int i = GC + (int)(t / d.ny), j = GC + (int)(t % d.ny);
+            const size_t o = d.off + (size_t)i * g.ld + j;
+#pragma unroll
+            for (int v = 0; v < 4; v++) a.pool[3][(size_t)v * g.plane + o] = a.pool[0][(size_t)v * g.plane + o];
+        }
+    }
+}
+
 // Clock reset and wave-speed maxima of problem k's current state (after set_state).
 __global__ void k_batch_reset(BatchArgs a, int k) {
     if (threadIdx.x == 0) {
@@ -949,6 +1170,7 @@ __global__ void k_batch_reset(BatchArgs a, int k) {
         s.steps = 0;
         s.active = 0;
         s.faults = 0u;
+        s.mirror = 0;
     }
 }
 
@@ -1004,6 +1226,9 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
             last = 1;
         }
     }
+    // with the safeguard the last stage of RK2 / RK3 writes pool[3] (a repair recomputes
+    // cells from U^n, which an in-place update would have overwritten)
+    if (a.positivity && time_order > 1 && last) oi = 3;
     if (l.staging == STAGING_SWEEPS) {
         // blocks cover the longest side of any problem; each side checks its own range
         const dim3 gg((a.max_side + 63) / 64, 4, a.nprob < 65535 ? a.nprob : 65535);
@@ -1026,9 +1251,12 @@ void stage_launch(const BatchArgs& a, int time_order, int stage, const BatchStag
         if constexpr (MIXED)
             hipLaunchKernelGGL((k_batch_fix<ORDER, CHAR>), dim3(1024), dim3(256), 0, s, a, stage,
                                kind, yi, oi, 0);
+        if (a.positivity)
+            hipLaunchKernelGGL((k_batch_repair<ORDER, CHAR, MIXED>), dim3(1), dim3(256), 0, s, a,
+                               stage, kind, yi, oi);
         // RK1 writes U1 and copies back afterwards: its maxima come after the copy
         if (last && time_order != 1)
-            launch_batch_speeds_all(a, s);
+            launch_batch_speeds_all(a, oi, s);
         return;
     } else if (l.staging == STAGING_LDS)
         hipLaunchKernelGGL((k_batch_stage<ORDER, CHAR, MIXED, STAGING_LDS>), dim3(l.grid), dim3(64),
@@ -1073,9 +1301,13 @@ void launch_batch_reset_speeds(const BatchArgs& a, int k, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a, k);
 }
 
-void launch_batch_speeds_all(const BatchArgs& a, hipStream_t s) {
+void launch_batch_speeds_all(const BatchArgs& a, int pi, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_speeds_all, dim3(64, a.nprob < 65535 ? a.nprob : 65535), dim3(256),
-                       0, s, a);
+                       0, s, a, pi);
+}
+
+void launch_batch_mirror(const BatchArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_mirror, dim3(64, a.nprob < 65535 ? a.nprob : 65535), dim3(256), 0, s, a);
 }
 
 void launch_batch_speeds(const BatchArgs& a, int k, hipStream_t s) {

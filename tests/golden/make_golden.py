@@ -379,13 +379,46 @@ def make_dmrbc():
     return out
 
 
+def state_digest(U):
+    """SHA-256 of a state's bytes with -0.0 read as +0.0 (the reference and the device
+    may differ in the sign of a zero momentum, which no comparison here counts)."""
+    import hashlib
+    A = np.ascontiguousarray(np.where(np.asarray(U) == 0.0, 0.0, U), dtype="<f8")
+    return hashlib.sha256(A.tobytes()).hexdigest()
+
+
+def make_fulllength():
+    """Whole runs of the reference's own time loop (src/RP_FVM.c:179-191, mode "run"):
+    Sedov 200^2 to t = 1.0 (WENO5-char + HLLC, RK3, CFL 0.5, outflow) and Lax-Liu 3 with
+    the interface at 0.8 on 512^2 to t = 0.3.  Stored: the input state, every dt, the
+    final time, the SHA-256 of the final array (ghosts included), and per-variable
+    interior sums and a sparse sample of the final interior for diagnosis."""
+    from oracle.pyoracle import harness_run
+    out = {}
+    for name, (cfg, U0), t_end, stride in (("sedov200", cases.sedov(200), 1.0, 8),
+                                           ("ll3_512", cases.lax_liu_3(512), 0.3, 16)):
+        dts, t, U, rec = harness_run(cfg, U0, t_end, threads=8)
+        print(f"{name}: {len(dts)} steps to t={t} in {rec['seconds']:.1f} s")
+        inner = U[:, 4:-4, 4:-4]
+        out[f"{name}_cfg"] = cfg_json(cfg)
+        out[f"{name}_U0"] = U0
+        out[f"{name}_t_end"] = np.array(t_end)
+        out[f"{name}_dts"] = dts
+        out[f"{name}_t"] = np.array(t)
+        out[f"{name}_digest"] = np.array(state_digest(U))
+        out[f"{name}_sums"] = inner.reshape(4, -1).sum(axis=1)
+        out[f"{name}_stride"] = np.array(stride)
+        out[f"{name}_sample"] = inner[:, ::stride, ::stride].copy()
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run `make -C oracle ref` first")
     makers = (("weno", make_weno), ("flux", make_flux), ("eigen", make_eigen),
               ("recon", make_recon), ("bc", make_bc), ("residual", make_residual),
               ("steps", make_steps), ("dt", make_dt), ("dmrbc", make_dmrbc),
-              ("flux_hc", make_flux_hc))
+              ("flux_hc", make_flux_hc), ("fulllength", make_fulllength))
     only = set(sys.argv[1:])  # fixture names to (re)write; all when empty
     for name, fn in makers:
         if only and name not in only:

@@ -76,6 +76,9 @@ def order_batch(precision, order):
 
 HC_SCHEMES = (11, 22, 33, 22, 33, 11)
 
+# states of the seeded generator away from near vacuum (rho >= 0.3, p >= 0.2, |u|, |v| <= 1)
+NARROW_RANGES = {"rho": (0.3, 2.0), "p": (0.2, 2.0), "uv": (-1.0, 1.0)}
+
 
 def hc_batch(characteristic):
     """small_batch with the heat-capacity solvers HLLHC/HLLCHC/RoeHC (FP64 batches)."""
@@ -247,9 +250,10 @@ def test_batch_rk_orders(staging):
 @pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_SWEEPS])
 @pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 def test_batch_seeded_workload_prefix(precision, staging):
-    """The first problems of the seeded 64-problem workload (sizes 256..1536), a few
-    steps: identical to the single-problem path."""
-    cfgs, states = cases.batch_case(count=64)
+    """The first problems of the seeded 64-problem generator (sizes 256..1536) with
+    states kept clear of near vacuum and no safeguard (so that every schedule runs them),
+    a few steps: identical to the single-problem path."""
+    cfgs, states = cases.batch_case(count=64, positivity=False, ranges=NARROW_RANGES)
     pick = [0, 1, 2, 3]
     cfgs = [cfgs[k] for k in pick]
     states = [states[k] for k in pick]
@@ -270,3 +274,130 @@ def test_batch_seeded_workload_prefix(precision, staging):
                 s.time_advance(s.get_delta_t())
             assert np.array_equal(got[k], interior(s.get_state()))
         assert steps[k] == 4
+
+
+# ----------------------------------------------------------- positivity safeguard
+def wide_batch(precision, seed=32, time_order=3):
+    """small_batch with the positivity safeguard and the seeded workload's state ranges
+    (rho ~ U[0.1,2], p ~ U[0.02,2], u, v ~ U[-1.5,1.5]), which reach near vacuum: the
+    safeguard repairs cells in several of these problems (pinned on the CPU by
+    test_batch_cases_cpu.py)."""
+    rng = np.random.default_rng(seed)
+    shapes = [(70, 51), (33, 129), (64, 64), (100, 17), (9, 40), (127, 66)]
+    schemes = [1, 2, 3, 1, 10, 1]
+    bcs = [(BC_OUTFLOW,) * 4, (BC_REFLECT,) * 4, (BC_PERIODIC,) * 4,
+           (BC_INFLOW, BC_OUTFLOW, BC_REFLECT, BC_FIXED), (BC_FIXED, BC_REFLECT, BC_OUTFLOW, BC_PERIODIC),
+           (BC_REFLECT, BC_OUTFLOW, BC_PERIODIC, BC_PERIODIC)]
+    cfgs, states = [], []
+    for (nx, ny), sch, bc in zip(shapes, schemes, bcs):
+        quad = {k: (float(rng.uniform(0.1, 2.0)), float(rng.uniform(-1.5, 1.5)),
+                    float(rng.uniform(-1.5, 1.5)), float(rng.uniform(0.02, 2.0)))
+                for k in ("NE", "NW", "SW", "SE")}
+        c, U = cases.four_quadrant(nx, ny, quad, x0=0.43, y0=0.57, perturb=1e-3, seed=int(nx),
+                                   scheme=sch, characteristic=True, bc=bc, cfl=0.45,
+                                   positivity=True, time_order=time_order)
+        c.precision = precision
+        c.inflow_state[:] = [1.0, 0.5, -0.2, 3.0]
+        for s in range(4):
+            c.fixed_state[s][:] = [0.9 + 0.1 * s, 0.1, 0.2, 2.5]
+        cfgs.append(c)
+        states.append(U)
+    return cfgs, states
+
+
+WIDE_T_END = 0.04
+
+
+@pytest.mark.parametrize("time_order", [3, 2, 1])
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
+def test_batch_safeguard_matches_single_problems(precision, time_order):
+    """With the positivity safeguard each problem is bit-identical to the safeguarded
+    single-problem path (same dts, step counts and fault bits), RK1 (copy back), RK2 and
+    RK3 (last stage into the fourth pool, swapped in)."""
+    cfgs, states = wide_batch(precision, time_order=time_order)
+    with Batch(cfgs) as b:
+        assert b.get_tuning()["staging"] == STAGING_SWEEPS
+        for k, U in enumerate(states):
+            b.set_state(k, U)
+        b.advance_to(WIDE_T_END, 100000)
+        t, steps, faults = b.status()
+        got = [b.get_state(k) for k in range(len(cfgs))]
+    for k, (c, U0) in enumerate(zip(cfgs, states)):
+        print(f"problem {k}: {c.nx}x{c.ny} scheme {c.scheme} order {time_order}", flush=True)
+        ref, n, f = single(c, U0, WIDE_T_END)
+        assert np.isfinite(interior(ref)).all(), k
+        assert (int(faults[k]) & 9, int(steps[k]), float(t[k])) == (f & 9, n, WIDE_T_END), k
+        assert np.array_equal(interior(got[k]), interior(ref)), k
+
+
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
+def test_batch_safeguard_graph_replay_matches_eager(precision):
+    """The two step graphs (one per pool assignment) replayed with host reads in between
+    give the eager launches' states at every step."""
+    cfgs, states = wide_batch(precision)
+    res = {}
+    for use_graph in (0, 1):
+        with Batch(cfgs) as b:
+            b.set_tuning(use_graph=use_graph)
+            for k, U in enumerate(states):
+                b.set_state(k, U)
+            snaps = []
+            for _ in range(5):
+                b.step(1.0)
+                snaps.append([interior(b.get_state(k)).copy() for k in range(len(cfgs))])
+                b.status()
+            res[use_graph] = snaps
+    for s0, s1 in zip(res[0], res[1]):
+        for a, c in zip(s0, s1):
+            assert np.array_equal(a, c)
+
+
+def test_batch_safeguard_schedule_and_tuning():
+    """The safeguard runs in the two-sweep schedule: the tuner keeps to it and a one-pass
+    schedule is refused; tuning leaves the state alone."""
+    from overheatingproblem_amd.capi import RpfvmError
+    cfgs, states = wide_batch(0)
+    with Batch(cfgs) as b:
+        with pytest.raises(RpfvmError, match="two-sweep"):
+            b.set_tuning(staging=STAGING_DIRECT)
+        for k, U in enumerate(states):
+            b.set_state(k, U)
+        b.step(WIDE_T_END)
+        tuned = b.tune()
+        assert tuned["staging"] == STAGING_SWEEPS
+        b.advance_to(WIDE_T_END, 100000)
+        got = [interior(b.get_state(k)) for k in range(len(cfgs))]
+    for k, (c, U0) in enumerate(zip(cfgs, states)):
+        ref, _, _ = single(c, U0, WIDE_T_END)
+        assert np.array_equal(got[k], interior(ref)), k
+
+
+@pytest.mark.parametrize("precision", [PRECISION_MIXED, 0])
+def test_batch_seeded_workload_all_problems(precision):
+    """Every problem of the seeded 64-problem workload (rho ~ U[0.1,2], p ~ U[0.02,2],
+    u, v ~ U[-1.5,1.5], sizes 256..1536, safeguard on) for 6 steps, bit-identical to the
+    safeguarded single-problem path; two of its problems need cell repairs from the first
+    steps on."""
+    cfgs, states = cases.batch_case(count=64)
+    for c in cfgs:
+        c.precision = precision
+    nsteps = 6
+    with Batch(cfgs) as b:
+        for k, U in enumerate(states):
+            b.set_state(k, U)
+        b.tune()
+        for _ in range(nsteps):
+            b.step(0.25)
+        t, steps, faults = b.status()
+        got = [interior(b.get_state(k)).copy() for k in range(len(cfgs))]
+    print(f"batch done: steps {set(steps.tolist())}, fault bits {set(faults.tolist())}", flush=True)
+    for k, (c, U0) in enumerate(zip(cfgs, states)):
+        with Solver(c) as s:
+            s.set_state(U0)
+            for _ in range(nsteps):
+                s.time_advance(s.get_delta_t())
+            ref = interior(s.get_state())
+            f = s.sync()
+        assert steps[k] == nsteps and (int(faults[k]) & 9) == (f & 9), k
+        assert np.isfinite(ref).all(), k
+        assert np.array_equal(got[k], ref), k

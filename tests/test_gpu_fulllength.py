@@ -1,0 +1,92 @@
+"""Whole runs of the FP64 form against the reference's own time loop.
+
+The fixtures (tests/golden/fulllength.npz, written by make_golden.py from the reference
+solver compiled from its sources) hold the input state, every dt and a SHA-256 of the
+final array of two complete runs of src/RP_FVM.c:179-191 (dt = Get_Delta_T_2D clipped to
+t_end, then RK3_TimeAd, include/time_advance.h:788-882):
+
+  * Sedov-type blast on [-1,1]^2, 200^2, WENO5-char + HLLC, CFL 0.5, outflow, t = 1.0
+    (3760 steps);
+  * Lax-Liu configuration 3 with the interface at (0.8, 0.8), 512^2, same scheme,
+    t = 0.3.
+
+The device must reproduce every dt and the final state bit for bit (a zero's sign
+aside).  On a mismatch the test reports the first step whose dt differs and the
+per-variable sums and sampled cells of the final interior.
+"""
+import numpy as np
+import pytest
+
+import golden_util as golden
+
+pytestmark = pytest.mark.gpu
+
+RUNS = ("sedov200", "ll3_512")
+
+
+def digest(U):
+    import hashlib
+    A = np.ascontiguousarray(np.where(np.asarray(U) == 0.0, 0.0, U), dtype="<f8")
+    return hashlib.sha256(A.tobytes()).hexdigest()
+
+
+def driver_loop(solver, t_end, max_steps):
+    """src/RP_FVM.c:179-191 with the next output time at t_end."""
+    t, dts = 0.0, []
+    while t < t_end and len(dts) < max_steps:
+        dt = solver.get_delta_t()
+        if t + dt > t_end:
+            dt = t_end - t
+        if t + dt >= t_end:
+            dt = t_end - t
+        dts.append(dt)
+        solver.time_advance(dt)
+        t = t + dt
+    return t, np.array(dts)
+
+
+def explain(d, name, U, dts):
+    ref = d[f"{name}_dts"]
+    n = min(len(ref), len(dts))
+    diff = np.nonzero(ref[:n] != dts[:n])[0]
+    inner = U[:, 4:-4, 4:-4]
+    st = int(d[f"{name}_stride"])
+    samp = inner[:, ::st, ::st]
+    return (f"{len(dts)} steps vs {len(ref)}; first dt difference at step "
+            f"{int(diff[0]) if len(diff) else None}; interior sums "
+            f"{inner.reshape(4, -1).sum(axis=1)} vs {d[f'{name}_sums']}; sampled cells "
+            f"differing: {int((samp != d[f'{name}_sample']).sum())} of {samp.size}")
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_full_run_bit_identical_to_reference(name):
+    from overheatingproblem_amd.capi import Solver
+    d = golden.load("fulllength")
+    cfg = golden.cfg(d, f"{name}_cfg")
+    t_end = float(d[f"{name}_t_end"])
+    ref_dts = d[f"{name}_dts"]
+    print(f"{name}: {cfg.nx}x{cfg.ny} to t={t_end}, {len(ref_dts)} reference steps", flush=True)
+    with Solver(cfg) as s:
+        s.set_state(d[f"{name}_U0"])
+        t, dts = driver_loop(s, t_end, len(ref_dts) + 8)
+        U = s.get_state()
+        faults = s.sync()
+    assert faults == 0
+    ok = (len(dts) == len(ref_dts) and np.array_equal(dts, ref_dts)
+          and t == float(d[f"{name}_t"]) and digest(U) == str(d[f"{name}_digest"]))
+    assert ok, explain(d, name, U, dts)
+
+
+def test_device_loop_matches_reference_sedov():
+    """rpfvm_advance_to (dt and clock on the device) over the whole Sedov run."""
+    from overheatingproblem_amd.capi import Solver
+    d = golden.load("fulllength")
+    name = "sedov200"
+    cfg = golden.cfg(d, f"{name}_cfg")
+    with Solver(cfg) as s:
+        s.set_state(d[f"{name}_U0"])
+        t, steps = s.advance_to(float(d[f"{name}_t_end"]), 10 ** 6)
+        U = s.get_state()
+        assert s.sync() == 0
+    assert steps == len(d[f"{name}_dts"]) and t == float(d[f"{name}_t"])
+    assert digest(U) == str(d[f"{name}_digest"]), explain(d, name, U, d[f"{name}_dts"])

@@ -188,3 +188,16 @@ def test_double_mach_sides(oracle, key):
     if key != "t0":
         top = ref[0, 4:R - 4, C - 4]
         assert (top == 8.0).any() and (top == 1.4).any()
+
+
+@pytest.mark.parametrize("name,steps", [("sedov200", 150), ("ll3_512", 60)])
+def test_full_run_prefix(oracle, name, steps):
+    """The first steps of the whole-run fixtures (the reference's own time loop): every
+    dt bit for bit.  dt is the CFL bound of the whole state after the previous step, so
+    a state that differs anywhere in its fastest cells shows here."""
+    d = load("fulllength")
+    c = golden_cfg(d, name + "_cfg")
+    U = np.array(d[name + "_U0"])
+    t, dts = oracle.advance_to(c, U, float(d[name + "_t_end"]), steps)
+    assert len(dts) == steps
+    assert np.array_equal(dts, d[name + "_dts"][:steps])
