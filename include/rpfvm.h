@@ -201,6 +201,25 @@ int rpfvm_get_time(rpfvm_ctx *ctx, double *t);
 int rpfvm_sync(rpfvm_ctx *ctx);
 int rpfvm_get_faults(rpfvm_ctx *ctx, int *faults);
 
+/* Get_Delta_T_2D on a caller's device state, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same reduction and dt as
+ * rpfvm_get_delta_t's pass over the state, bit for bit, without a context: U is the
+ * device address of plane 0, row 0, column 0 of a state laid out as four planes of
+ * nx + 8 rows with a row pitch of ld doubles (ld >= ny + 8) and plane v at
+ * U + v * plane_stride; dt (device) receives dt = CFL*min(dx,dy)/(S_x+S_y), clipped like
+ * the driver (src/RP_FVM.c:186-191) against clock[0] = t, clock[1] = t_stop when clock
+ * (device, two doubles) is not NULL; faults (device, optional) is OR-ed with
+ * RPFVM_FAULT_NONFINITE_DT when dt is not positive and finite (the caller clears it).
+ * Only cfg's nx, ny, dx, dy, gamma and cfl are read.  The launches go on `stream` (a
+ * hipStream_t, NULL for the default stream) and the call returns without waiting.
+ * Scratch: a fixed device array of 2048 workgroup maxima pairs (32 KiB) per GPU, held
+ * until the process ends; any grid the configuration admits fits in it.  Calls that
+ * overlap on one GPU (other streams or host threads) share that array and race. */
+int rpfvm_delta_t_noalloc(const rpfvm_config *cfg, const double *U, size_t ld,
+                          size_t plane_stride, const double *clock, double *dt,
+                          unsigned *faults, void *stream);
+
 /* Device access for callers that keep data on the GPU.  The state lives in four
  * planes of R rows with a row pitch of *ld doubles (ld >= C); plane v starts at
  * base + v * plane_stride.  The stream is a hipStream_t.  The next dt is normally

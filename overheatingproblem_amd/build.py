@@ -35,7 +35,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract
 EXTRA = os.environ.get("RPFVM_HIPFLAGS", "").split()
 
 SOURCES = ["fvm_kernels.hip", "fvm_mixed_kernels.hip", "fvm_batch_kernels.hip", "fvm_api.cpp",
-           "fvm_batch_api.cpp"]
+           "fvm_batch_api.cpp", "fvm_noalloc.cpp"]
 
 
 def _compile(src: str) -> str:

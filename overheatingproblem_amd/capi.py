@@ -39,6 +39,7 @@ EXPORTS = (
     "rpfvm_batch_create", "rpfvm_batch_destroy", "rpfvm_batch_set_state", "rpfvm_batch_get_state",
     "rpfvm_batch_step", "rpfvm_batch_advance_to", "rpfvm_batch_status", "rpfvm_batch_sync",
     "rpfvm_batch_tune", "rpfvm_batch_set_tuning", "rpfvm_batch_get_tuning", "rpfvm_batch_stream",
+    "rpfvm_delta_t_noalloc",
 )
 
 STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS = 0, 1, 2
@@ -123,6 +124,8 @@ def lib():
     L.rpfvm_batch_set_tuning.argtypes = [vp, tp]
     L.rpfvm_batch_get_tuning.argtypes = [vp, tp]
     L.rpfvm_batch_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.rpfvm_delta_t_noalloc.argtypes = [ctypes.POINTER(Config), vp, ctypes.c_size_t, ctypes.c_size_t,
+                                        vp, vp, vp, vp]
     L.rpfvm_last_error.restype = ctypes.c_char_p
     L.rpfvm_version.restype = ctypes.c_char_p
     for name in EXPORTS:
