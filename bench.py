@@ -346,14 +346,108 @@ def noh_metrics(states, n, gamma, t_end=0.6):
     return out
 
 
-def noh_full(n, t_end=0.6):
-    """Both forms to t_end on the Noh workload, with the accuracy report of
-    :func:`noh_metrics`.  Four runs: FP64 (its dt sequence recorded), mixed as it runs
-    (its own dts), mixed replaying the FP64 dts (the arithmetic difference alone), and
-    FP64 from a state perturbed by 1e-12 relative (the problem's own sensitivity to
-    round-off, the yardstick for the other two)."""
+def noh_regions(n, t):
+    """Masks of the Noh interior (cell centres) where round-off grows: the origin corner
+    r < 0.05, bands of 0.02 along the two reflecting walls, and a band of +-0.01 around the
+    exact shock radius t/3; "rest" is everything else."""
     from overheatingproblem_amd import cases
+    X, Y = cases.cell_centres(n, n, 1.0 / n, 1.0 / n, ghosts=False)
+    r = np.sqrt(X * X + Y * Y)
+    corner = r < 0.05
+    axes = ((X < 0.02) | (Y < 0.02)) & ~corner
+    shock = (np.abs(r - t / 3.0) < 0.01) & ~corner & ~axes
+    rest = ~(corner | axes | shock)
+    return {"corner": corner, "axes": axes, "shock": shock, "rest": rest}
+
+
+def region_l1(a, b, masks):
+    """L1 (mean |a - b|) over the whole interior and over each region, each region's share
+    of the total, and the interior cell (i, j) of the largest difference."""
+    e = np.abs(a - b)
+    tot = float(e.sum())
+    at = np.unravel_index(int(np.argmax(e)), e.shape)
+    out = {"l1": float(e.mean()), "linf": float(e.max()), "linf_at": [int(at[0]), int(at[1])]}
+    for k, m in masks.items():
+        out[f"l1_{k}"] = float(e[m].mean()) if m.any() else 0.0
+        out[f"share_{k}"] = float(e[m].sum() / tot) if tot > 0 else 0.0
+    out["l1_masked"] = out["l1_rest"]
+    return out
+
+
+def lockstep_history(cfg, U0, t_end, others, every, label):
+    """Run the FP64 form with cfg through the driver loop (src/RP_FVM.c:179-191) and every
+    (name, cfg, U0) of `others` on the same dts in lockstep; every `every` steps (and at
+    the end) compare each other run's density with the FP64 run's by region.  Returns
+    (history, final states, FP64 dts, wall time)."""
     from overheatingproblem_amd.capi import Solver
+    solvers = {"fp64": Solver(cfg)}
+    solvers["fp64"].set_state(U0)
+    for name, c, U in others:
+        solvers[name] = Solver(c)
+        solvers[name].set_state(U)
+    hist, dts, t, steps = [], [], 0.0, 0
+    t0 = time.perf_counter()
+    try:
+        def snapshot():
+            U64 = solvers["fp64"].get_state()
+            r64 = cases_cons_to_prim(U64, cfg.gamma)[0]
+            row = {"step": steps, "t": t}
+            masks = label["regions"](t, r64)
+            for name in solvers:
+                if name == "fp64":
+                    continue
+                rx = cases_cons_to_prim(solvers[name].get_state(), cfg.gamma)[0]
+                row[name] = region_l1(rx, r64, masks)
+            hist.append(row)
+            log(f"bench: {label['name']} step {steps} t={t:.4f}: " +
+                ", ".join(f"{k} L1 {row[k]['l1']:.3e} (rest {row[k]['l1_rest']:.3e})"
+                          for k in solvers if k != "fp64"))
+        while t < t_end:
+            dt = solvers["fp64"].get_delta_t()
+            if t + dt > t_end:
+                dt = t_end - t
+            if t + dt >= t_end:
+                dt = t_end - t
+            for s in solvers.values():
+                s.time_advance(dt)
+            dts.append(dt)
+            t = t + dt
+            steps += 1
+            if steps % every == 0:
+                snapshot()
+        if steps % every:
+            snapshot()
+        finals = {k: s.get_state() for k, s in solvers.items()}
+        faults = {k: s.sync() for k, s in solvers.items()}
+    finally:
+        for s in solvers.values():
+            s.close()
+    return hist, finals, faults, dts, time.perf_counter() - t0
+
+
+def cases_cons_to_prim(U, gamma):
+    from overheatingproblem_amd.cases import cons_to_prim
+    return cons_to_prim(U[:, 4:-4, 4:-4], gamma)
+
+
+def run_own_dts(cfg, U0, t_end):
+    from overheatingproblem_amd.capi import Solver
+    t0 = time.perf_counter()
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        t, steps = s.advance_to(t_end, 10 ** 7)
+        return s.get_state(), steps, s.sync(), time.perf_counter() - t0
+
+
+def noh_full(n, t_end=0.6, every=500):
+    """Both forms to t_end on the Noh workload.  FP64 runs the driver loop; FP64 from a
+    density perturbed by 1e-12 (relative) and the mixed form run on the FP64 run's dts in
+    lockstep, and every `every` steps their densities are compared with FP64's over the
+    whole grid and by region (origin corner r < 0.05, wall bands, shock band, rest):
+    where round-off grows, and the metric's L1 on the part of the grid where FP64 itself
+    is round-off stable.  Then the mixed form once more on its own dts.  Wall-heating
+    figures of every final state as :func:`noh_metrics`."""
+    from overheatingproblem_amd import cases
     from overheatingproblem_amd.config import PRECISION_MIXED
     cfg, U0 = cases.noh(n, positivity=True)
     mx = cfg.copy()
@@ -361,48 +455,87 @@ def noh_full(n, t_end=0.6):
     rng = np.random.default_rng(12)
     Up = U0.copy()
     Up[0] *= 1.0 + 1e-12 * rng.uniform(-1.0, 1.0, size=Up[0].shape)
-
-    def run(c, U, dts=None, record=False):
-        t0 = time.perf_counter()
-        used = []
-        with Solver(c) as s:
-            s.set_state(U)
-            if dts is None and not record:
-                t, steps = s.advance_to(t_end, 10 ** 7)
-            else:
-                t, steps = 0.0, 0
-                while (t < t_end) if dts is None else (steps < len(dts)):
-                    if dts is None:  # the driver loop, src/RP_FVM.c:179-191
-                        dt = s.get_delta_t()
-                        if t + dt > t_end:
-                            dt = t_end - t
-                        if t + dt >= t_end:
-                            dt = t_end - t
-                    else:
-                        dt = dts[steps]
-                    used.append(dt)
-                    s.time_advance(dt)
-                    t = t + dt
-                    steps += 1
-            out = {"U": s.get_state(), "t": t, "steps": steps, "faults": s.sync(),
-                   "wall_s": time.perf_counter() - t0, "dts": used}
-        log(f"bench: noh {n}^2 to t={t} in {steps} steps, {out['wall_s']:.1f} s")
-        return out
-
-    res = {"fp64": run(cfg, U0, record=True)}
-    res["mixed"] = run(mx, U0)
-    res["mixed_replay"] = run(mx, U0, dts=res["fp64"]["dts"])
-    res["fp64_perturbed"] = run(cfg, Up)
-    out = {"workload": f"noh-{n}x{n}", "t_end": t_end}
-    m = noh_metrics({"fp64": res["fp64"]["U"], "mixed": res["mixed"]["U"]}, n, cfg.gamma, t_end)
+    label = {"name": f"noh {n}^2", "regions": lambda t, r64: noh_regions(n, t)}
+    hist, finals, faults, dts, wall = lockstep_history(
+        cfg, U0, t_end, [("mixed_replay", mx, U0), ("fp64_perturbed", cfg, Up)], every, label)
+    log(f"bench: noh {n}^2 lockstep done: {len(dts)} steps, {wall:.1f} s; mixed on its own dts ...")
+    Umx, mx_steps, mx_faults, mx_wall = run_own_dts(mx, U0, t_end)
+    finals["mixed"] = Umx
+    out = {"workload": f"noh-{n}x{n}", "t_end": t_end, "every": every,
+           "regions": "corner r<0.05; axes x<0.02 or y<0.02; shock |r-t/3|<0.01; rest",
+           "history": hist}
+    m = noh_metrics({"fp64": finals["fp64"], "mixed": finals["mixed"]}, n, cfg.gamma, t_end)
     out.update(m)
     for other in ("mixed_replay", "fp64_perturbed"):
-        mm = noh_metrics({"fp64": res["fp64"]["U"], "mixed": res[other]["U"]}, n, cfg.gamma, t_end)
+        mm = noh_metrics({"fp64": finals["fp64"], "mixed": finals[other]}, n, cfg.gamma, t_end)
         out[other] = mm["mixed"]
         out[other + "_vs_fp64"] = mm["mixed_vs_fp64"]
-    for k, v in res.items():
-        out[k].update({"steps": v["steps"], "faults": v["faults"], "wall_s": round(v["wall_s"], 2),
-                       "t": v["t"]})
+    masks = noh_regions(n, t_end)
+    r64 = cases_cons_to_prim(finals["fp64"], cfg.gamma)[0]
+    out["final_regions"] = {k: region_l1(cases_cons_to_prim(finals[k], cfg.gamma)[0], r64, masks)
+                            for k in ("mixed", "mixed_replay", "fp64_perturbed")}
+    out["fp64"].update({"steps": len(dts), "faults": faults["fp64"], "wall_s": round(wall, 2), "t": t_end})
+    out["mixed_replay"].update({"steps": len(dts), "faults": faults["mixed_replay"]})
+    out["fp64_perturbed"].update({"steps": len(dts), "faults": faults["fp64_perturbed"]})
+    out["mixed"].update({"steps": mx_steps, "faults": mx_faults, "wall_s": round(mx_wall, 2), "t": t_end})
+    return out
+
+
+def dmr_regions(rho):
+    """The double-Mach-reflection interior split at its discontinuities: cells whose
+    density differs from a neighbour's by more than 10% in the FP64 state are the
+    'fronts' (shocks, slip line, jet) and the rest is smooth flow."""
+    d = np.zeros_like(rho, dtype=bool)
+    jx = np.abs(np.diff(rho, axis=0)) > 0.1 * np.minimum(rho[1:], rho[:-1])
+    jy = np.abs(np.diff(rho, axis=1)) > 0.1 * np.minimum(rho[:, 1:], rho[:, :-1])
+    d[1:] |= jx
+    d[:-1] |= jx
+    d[:, 1:] |= jy
+    d[:, :-1] |= jy
+    return {"fronts": d, "rest": ~d}
+
+
+def dmr_full(nx=8192, ny=2048, t_end=0.2, every=2000):
+    """Mach-10 double Mach reflection (moving boundaries, WENO5-char + Roe with the HLL(E)
+    positivity fallback, CFL 0.5) to t_end: FP64 through the driver loop; FP64 from a
+    density perturbed by 1e-12 (relative) and the mixed form on its dts in lockstep
+    (density compared with FP64's every `every` steps, over the whole grid and split into
+    fronts and smooth flow); then the mixed form on its own dts (its throughput over the
+    whole run)."""
+    from overheatingproblem_amd import cases
+    from overheatingproblem_amd.config import PRECISION_MIXED
+    cfg, U0 = cases.dmr(nx, ny)
+    mx = cfg.copy()
+    mx.precision = PRECISION_MIXED
+    rng = np.random.default_rng(12)
+    Up = U0.copy()
+    Up[0] *= 1.0 + 1e-12 * rng.uniform(-1.0, 1.0, size=Up[0].shape)
+    label = {"name": f"dmr {nx}x{ny}", "regions": lambda t, r64: dmr_regions(r64)}
+    hist, finals, faults, dts, wall = lockstep_history(
+        cfg, U0, t_end, [("mixed_replay", mx, U0), ("fp64_perturbed", cfg, Up)], every, label)
+    log(f"bench: dmr lockstep done: {len(dts)} steps, {wall:.1f} s; mixed on its own dts ...")
+    Umx, mx_steps, mx_faults, mx_wall = run_own_dts(mx, U0, t_end)
+    g = cfg.gamma
+    r64, p64 = (cases_cons_to_prim(finals["fp64"], g)[k] for k in (0, 3))
+    masks = dmr_regions(r64)
+    cells = nx * ny
+    out = {"workload": f"dmr-{nx}x{ny}", "t_end": t_end, "every": every, "history": hist,
+           "regions": "fronts: cells whose density differs from a neighbour's by > 10% in the "
+                      "FP64 state; rest: smooth flow",
+           "fp64": {"steps": len(dts), "faults": faults["fp64"], "lockstep_wall_s": round(wall, 2)},
+           "mixed_replay": {"faults": faults["mixed_replay"]},
+           "fp64_perturbed": {"faults": faults["fp64_perturbed"]},
+           "mixed": {"steps": mx_steps, "faults": mx_faults, "wall_s": round(mx_wall, 2),
+                     "mcell_updates_per_s": round(cells * mx_steps / mx_wall / 1e6, 1)},
+           "final_regions": {k: region_l1(cases_cons_to_prim(U, g)[0], r64, masks)
+                             for k, U in (("mixed_replay", finals["mixed_replay"]),
+                                          ("fp64_perturbed", finals["fp64_perturbed"]),
+                                          ("mixed", Umx))},
+           "final_p": {k: {"l1": float(np.abs(cases_cons_to_prim(U, g)[3] - p64).mean()),
+                           "linf": float(np.abs(cases_cons_to_prim(U, g)[3] - p64).max())}
+                       for k, U in (("mixed_replay", finals["mixed_replay"]),
+                                    ("fp64_perturbed", finals["fp64_perturbed"]), ("mixed", Umx))},
+           "fronts_fraction": float(masks["fronts"].mean())}
     return out
 
 
@@ -464,34 +597,46 @@ SOLVERS = ((2, "HLLC"), (1, "HLL"), (3, "Roe"), (10, "LLF"), (11, "HLLHC"), (22,
            (33, "RoeHC"))
 
 
-def noh_solvers(n, t_end=0.6):
-    """Noh to t_end in the FP64 form with the positivity safeguard, once per Riemann solver
-    of the reference (schemes 1, 2, 3, 10 and the heat-capacity variants 11, 22, 33, which
-    target exactly the wall heating this workload shows): the density deficit each leaves
-    at the wall, the plateau, its fault bits and its Mcell-updates/s.  At most 6 n steps
-    (the run needs about 4 n), so a solver whose state goes non-finite stops early."""
+def noh_solvers(n, t_end=0.6, forms=("fp64", "mixed")):
+    """Noh to t_end with the positivity safeguard, once per Riemann solver of the reference
+    (schemes 1, 2, 3, 10 and the heat-capacity variants 11, 22, 33, which target exactly
+    the wall heating this workload shows) and per form: the density deficit each leaves at
+    the wall, the plateau, its fault bits and its Mcell-updates/s, and for the mixed form
+    its L1(rho) from the FP64 run of the same solver.  At most 6 n steps (the run needs
+    about 4 n), so a solver whose state goes non-finite stops early."""
     from overheatingproblem_amd import cases
     from overheatingproblem_amd.capi import Solver
+    from overheatingproblem_amd.config import PRECISION_MIXED
     rows = []
     for scheme, name in SOLVERS:
-        cfg, U0 = cases.noh(n, positivity=True, scheme=scheme)
-        with Solver(cfg) as s:
-            s.set_state(U0)
-            t0 = time.perf_counter()
-            t, steps = s.advance_to(t_end, 6 * n)
-            faults = s.sync()
-            wall = time.perf_counter() - t0
-            U = s.get_state()
-        finite = bool(np.isfinite(U).all())
-        row = {"scheme": scheme, "solver": name, "steps": steps, "t": t, "faults": faults,
-               "finite": finite, "wall_s": round(wall, 2),
-               "mcell_updates_per_s": round(n * n * steps / wall / 1e6, 1)}
-        if finite:
-            row.update(noh_metrics({"fp64": U}, n, cfg.gamma, t_end)["fp64"])
-        log(f"bench: noh {n}^2 {name}: t={t:.4f} in {steps} steps, {wall:.1f} s, faults {faults}, "
-            f"finite {finite}")
-        rows.append(row)
-    return {"workload": f"noh-{n}x{n}", "t_end": t_end, "form": "fp64",
+        cfg0, U0 = cases.noh(n, positivity=True, scheme=scheme)
+        states = {}
+        for form in forms:
+            cfg = cfg0.copy()
+            if form == "mixed":
+                cfg.precision = PRECISION_MIXED
+            with Solver(cfg) as s:
+                s.set_state(U0)
+                t0 = time.perf_counter()
+                t, steps = s.advance_to(t_end, 6 * n)
+                faults = s.sync()
+                wall = time.perf_counter() - t0
+                U = s.get_state()
+            finite = bool(np.isfinite(U).all())
+            row = {"scheme": scheme, "solver": name, "form": form, "steps": steps, "t": t,
+                   "faults": faults, "finite": finite, "wall_s": round(wall, 2),
+                   "mcell_updates_per_s": round(n * n * steps / wall / 1e6, 1)}
+            if finite:
+                row.update(noh_metrics({"fp64": U}, n, cfg.gamma, t_end)["fp64"])
+                states[form] = U
+            if form == "mixed" and "fp64" in states and finite:
+                r64 = cases.cons_to_prim(states["fp64"][:, 4:-4, 4:-4], cfg.gamma)[0]
+                rmx = cases.cons_to_prim(U[:, 4:-4, 4:-4], cfg.gamma)[0]
+                row["l1_rho_vs_fp64"] = float(np.abs(rmx - r64).mean())
+            log(f"bench: noh {n}^2 {name} {form}: t={t:.4f} in {steps} steps, {wall:.1f} s, "
+                f"faults {faults}, finite {finite}")
+            rows.append(row)
+    return {"workload": f"noh-{n}x{n}", "t_end": t_end, "forms": list(forms),
             "positivity_safeguard": True, "solvers": rows}
 
 
@@ -510,6 +655,9 @@ def main():
     ap.add_argument("--ll3-full", action="store_true",
                     help="run Lax-Liu 3 at 2048^2 to t = 0.3 in both forms and print the "
                          "mixed-vs-FP64 accuracy instead")
+    ap.add_argument("--dmr-full", action="store_true",
+                    help="run the double Mach reflection 8192x2048 to t = 0.2 in both forms and "
+                         "print the accuracy record instead")
     ap.add_argument("--c1-full", action="store_true",
                     help="run the Sedov 200^2 workload to t = 1.0 with the reference on the host "
                          "cores and the FP64 form on the GPU, and print that record instead")
@@ -527,6 +675,9 @@ def main():
         return
     if args.noh_solvers:
         print(json.dumps(noh_solvers(args.n)), flush=True)
+        return
+    if args.dmr_full:
+        print(json.dumps(dmr_full()), flush=True)
         return
     if args.c1_full:
         print(json.dumps(c1_full(host_cores())), flush=True)

@@ -53,7 +53,8 @@ enum {
 /* Riemann solver ids as the "scheme" control key, include/CFD_convection.h:89-117.
  * Any other id except 5 selects LLF, as the reference's default branch.  The
  * heat-capacity solvers 11, 22, 33 (HLLHC_Flux, HLLCHC_Flux, RoeHC_Flux,
- * include/scheme.h:1860, 2153, 2793) run in the FP64 form only. */
+ * include/scheme.h:1860, 2153, 2793) run in both forms of a context and in FP64
+ * batches (not in mixed batches). */
 enum {
     RPFVM_SCHEME_HLL = 1, RPFVM_SCHEME_HLLC = 2, RPFVM_SCHEME_ROE = 3, RPFVM_SCHEME_LLF = 10,
     RPFVM_SCHEME_HLLHC = 11, RPFVM_SCHEME_HLLCHC = 22, RPFVM_SCHEME_ROEHC = 33

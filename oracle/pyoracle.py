@@ -191,8 +191,8 @@ def harness_run(cfg, U, t_end, *, max_steps=0, threads=None):
             f.write(harness_job("run", cfg, t_end=float(t_end), steps=int(max_steps)))
         U.tofile(os.path.join(td, "in.bin"))
         r = subprocess.run([REF_HARNESS, job, os.path.join(td, "in.bin"),
-                            os.path.join(td, "out.bin")], capture_output=True, text=True,
-                           env=env, check=True)
+                            os.path.join(td, "out.bin")], stdout=subprocess.PIPE, text=True,
+                           env=env, check=True)  # its progress lines go to our stderr
         raw = np.fromfile(os.path.join(td, "out.bin"), dtype="<f8")
     n, t = int(raw[0]), float(raw[1])
     rec = json.loads(r.stdout.strip().splitlines()[-1])

@@ -245,6 +245,7 @@ int main(int argc, char **argv) {
                 if (!dts) die("oom");
             }
             dts[n++] = Delta_T;
+            if (n % 500 == 0) fprintf(stderr, "ref_harness run: step %zu t %.6f\n", n, Time + Delta_T);
         }
         printf("{\"seconds\": %.9f, \"steps\": %zu, \"t\": %.17g, \"cells\": %lld, \"threads\": %d}\n",
                wall, n, Time, (long long)j.nx * j.ny, omp_get_max_threads());

@@ -16,7 +16,7 @@ BC_NAMES = {"outflow": BC_OUTFLOW, "reflection": BC_REFLECTION, "periodicity": B
             "inflow": BC_INFLOW, "fixed": BC_FIXED_VALUE}
 
 SCHEME_HLL, SCHEME_HLLC, SCHEME_ROE, SCHEME_LLF = 1, 2, 3, 10
-SCHEME_HLLHC, SCHEME_HLLCHC, SCHEME_ROEHC = 11, 22, 33   # FP64 form only
+SCHEME_HLLHC, SCHEME_HLLCHC, SCHEME_ROEHC = 11, 22, 33   # not in mixed batches
 # Solver names used in output file names (src/RP_FVM.c:785-793).
 SCHEME_NAMES = {1: "HLL", 2: "HLLC", 3: "Roe", 11: "HLLHC", 22: "HLLCHC", 33: "RoeHC"}
 

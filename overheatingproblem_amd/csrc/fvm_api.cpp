@@ -124,11 +124,6 @@ int fvm::api::validate(const rpfvm_config* c) {
         return fail(RPFVM_ERR_UNSUPPORTED, "scheme 5 (the Marquina stub) is not implemented");
     if (c->precision != RPFVM_PRECISION_FP64 && c->precision != RPFVM_PRECISION_MIXED)
         return fail(RPFVM_ERR_INVALID, "precision must be 0 (FP64) or 1 (mixed)");
-    if (c->precision == RPFVM_PRECISION_MIXED &&
-        (c->scheme == RPFVM_SCHEME_HLLHC || c->scheme == RPFVM_SCHEME_HLLCHC ||
-         c->scheme == RPFVM_SCHEME_ROEHC))
-        return fail(RPFVM_ERR_UNSUPPORTED,
-                    "the heat-capacity schemes 11, 22, 33 run in the FP64 form only");
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 6) return fail(RPFVM_ERR_INVALID, "bad bc[%d]", s);
     if (c->bc[0] >= RPFVM_BC_DMR_WALL || c->bc[1] >= RPFVM_BC_DMR_WALL ||

@@ -44,33 +44,64 @@ __device__ __forceinline__ double sqrt64(double x) {
 }
 
 // ---------------------------------------------------------------- FP32 WENO etc.
-__device__ __forceinline__ float weno5_f(float v1, float v2, float v3, float v4, float v5, bool z) {
-    const float eps = 1.0e-15f;
-    // second differences and one-sided first differences of the three stencils
-    const float t1 = fmaf(-2.0f, v2, v1) + v3, t2 = fmaf(3.0f, v3, fmaf(-4.0f, v2, v1));
-    const float t3 = fmaf(-2.0f, v3, v2) + v4, t4 = v2 - v4;
-    const float t5 = fmaf(-2.0f, v4, v3) + v5, t6 = fmaf(3.0f, v3, fmaf(-4.0f, v4, v5));
-    const float b1 = fmaf((13.0f / 12.0f) * t1, t1, (0.25f * t2) * t2);
-    const float b2 = fmaf((13.0f / 12.0f) * t3, t3, (0.25f * t4) * t4);
-    const float b3 = fmaf((13.0f / 12.0f) * t5, t5, (0.25f * t6) * t6);
-    float a1, a2, a3;
+// FP32 WENO5 of both face values of one field from its six-cell stencil s[0..5] (cells
+// i-2..i+3): the left value from (v1..v5) = (s0..s4), the right one from the mirrored
+// (s5..s1).  Per side: Jiang-Shu smoothness indicators
+// b = 13/12 (second difference)^2 + 1/4 (first difference)^2 + eps of the three
+// sub-stencils, weights d_k (1 + tau / b_k) with tau = |b_1 - b_3| (WENO5-Z, exponent 1;
+// z) or d_k / b_k^2 (WENO5-JS), d = (0.1, 0.6, 0.3), normalised, and the candidates
+// (2 v1 - 7 v2 + 11 v3, -v2 + 5 v3 + 2 v4, 2 v3 + 5 v4 - v5) / 6.  The two sides share
+// what their sub-stencils share: the second differences of (s1, s2, s3) and (s2, s3, s4)
+// and the candidates -s1 + 5 s2 + 2 s3 and 2 s2 + 5 s3 - s4 (the right side's third and
+// second).  eps rides in the first FMA of each indicator.  84 FP32 operations per field;
+// the two sides computed apart take 102.
+__device__ __forceinline__ void weno5_pair_f(const float s[6], bool z, float& left, float& right) {
+    const float eps = 1.0e-15f, c13 = 13.0f / 12.0f;
+    // second differences of (s0,s1,s2), (s1,s2,s3), (s2,s3,s4), (s3,s4,s5)
+    const float d0 = fmaf(-2.0f, s[1], s[0]) + s[2];
+    const float d1 = fmaf(-2.0f, s[2], s[1]) + s[3];
+    const float d2 = fmaf(-2.0f, s[3], s[2]) + s[4];
+    const float d3 = fmaf(-2.0f, s[4], s[3]) + s[5];
+    const float q0 = fmaf(c13 * d0, d0, eps), q1 = fmaf(c13 * d1, d1, eps);
+    const float q2 = fmaf(c13 * d2, d2, eps), q3 = fmaf(c13 * d3, d3, eps);
+    // one-sided first differences: left stencils 1..3, right stencils 1..3
+    const float e1 = fmaf(3.0f, s[2], fmaf(-4.0f, s[1], s[0])), e2 = s[1] - s[3];
+    const float e3 = fmaf(3.0f, s[2], fmaf(-4.0f, s[3], s[4]));
+    const float f1 = fmaf(3.0f, s[3], fmaf(-4.0f, s[4], s[5])), f2 = s[4] - s[2];
+    const float f3 = fmaf(3.0f, s[3], fmaf(-4.0f, s[2], s[1]));
+    // left: (s0,s1,s2), (s1,s2,s3), (s2,s3,s4); right: (s5,s4,s3), (s4,s3,s2), (s3,s2,s1)
+    const float bl1 = fmaf(0.25f * e1, e1, q0), bl2 = fmaf(0.25f * e2, e2, q1);
+    const float bl3 = fmaf(0.25f * e3, e3, q2);
+    const float br1 = fmaf(0.25f * f1, f1, q3), br2 = fmaf(0.25f * f2, f2, q2);
+    const float br3 = fmaf(0.25f * f3, f3, q1);
+    // candidates: left p1 p2 p3, right p1' (p2' = p3, p3' = p2)
+    const float pl1 = fmaf(2.0f, s[0], fmaf(-7.0f, s[1], 11.0f * s[2]));
+    const float p2 = fmaf(5.0f, s[2], fmaf(2.0f, s[3], -s[1]));
+    const float p3 = fmaf(2.0f, s[2], fmaf(5.0f, s[3], -s[4]));
+    const float pr1 = fmaf(2.0f, s[5], fmaf(-7.0f, s[4], 11.0f * s[3]));
+    float al1, al2, al3, ar1, ar2, ar3;
     if (z) {  // WENO5-Z, exponent 1
-        const float tau = fabsf(b1 - b3);
-        a1 = fmaf(0.1f * tau, rcp(b1 + eps), 0.1f);
-        a2 = fmaf(0.6f * tau, rcp(b2 + eps), 0.6f);
-        a3 = fmaf(0.3f * tau, rcp(b3 + eps), 0.3f);
+        const float tl = fabsf(bl1 - bl3), tr = fabsf(br1 - br3);
+        al1 = fmaf(0.1f * tl, rcp(bl1), 0.1f);
+        al2 = fmaf(0.6f * tl, rcp(bl2), 0.6f);
+        al3 = fmaf(0.3f * tl, rcp(bl3), 0.3f);
+        ar1 = fmaf(0.1f * tr, rcp(br1), 0.1f);
+        ar2 = fmaf(0.6f * tr, rcp(br2), 0.6f);
+        ar3 = fmaf(0.3f * tr, rcp(br3), 0.3f);
     } else {  // WENO5-JS
-        const float i1 = rcp(eps + b1), i2 = rcp(eps + b2), i3 = rcp(eps + b3);
-        a1 = (0.1f * i1) * i1;
-        a2 = (0.6f * i2) * i2;
-        a3 = (0.3f * i3) * i3;
+        const float il1 = rcp(bl1), il2 = rcp(bl2), il3 = rcp(bl3);
+        const float ir1 = rcp(br1), ir2 = rcp(br2), ir3 = rcp(br3);
+        al1 = (0.1f * il1) * il1;
+        al2 = (0.6f * il2) * il2;
+        al3 = (0.3f * il3) * il3;
+        ar1 = (0.1f * ir1) * ir1;
+        ar2 = (0.6f * ir2) * ir2;
+        ar3 = (0.3f * ir3) * ir3;
     }
-    const float is = rcp(a1 + a2 + a3) * (1.0f / 6.0f);
-    // candidate stencils 2 v1 - 7 v2 + 11 v3, -v2 + 5 v3 + 2 v4, 2 v3 + 5 v4 - v5
-    const float p1 = fmaf(2.0f, v1, fmaf(-7.0f, v2, 11.0f * v3));
-    const float p2 = fmaf(5.0f, v3, fmaf(2.0f, v4, -v2));
-    const float p3 = fmaf(2.0f, v3, fmaf(5.0f, v4, -v5));
-    return fmaf(a1, p1, fmaf(a2, p2, a3 * p3)) * is;
+    const float isl = rcp(al1 + al2 + al3) * (1.0f / 6.0f);
+    const float isr = rcp(ar1 + ar2 + ar3) * (1.0f / 6.0f);
+    left = fmaf(al1, pl1, fmaf(al2, p2, al3 * p3)) * isl;
+    right = fmaf(ar1, pr1, fmaf(ar2, p3, ar3 * p2)) * isr;
 }
 
 __device__ __forceinline__ float weno3_f(float v1, float v2, float v3) {
@@ -106,8 +137,7 @@ __device__ __forceinline__ void recon_scalar_f(const float s[6], float delta, fl
         l = weno3_f(s[1], s[2], s[3]);
         r = weno3_f(s[4], s[3], s[2]);
     } else {
-        l = weno5_f(s[0], s[1], s[2], s[3], s[4], CHAR);
-        r = weno5_f(s[5], s[4], s[3], s[2], s[1], CHAR);
+        weno5_pair_f(s, CHAR, l, r);
     }
 }
 
@@ -269,6 +299,12 @@ __device__ __forceinline__ void phys64(const double c[4], const Prim64& s, doubl
     F[3] = (c[3] + s.p) * q;
 }
 
+// The jump of the specific internal energy e = p / (rho (gamma - 1)) across the face, in
+// FP32 from the FP64 primitives (the heat-capacity term of HLLHC / HLLCHC / RoeHC).
+__device__ __forceinline__ float energy_jump(double g, const Prim64& L, const Prim64& R) {
+    return (float)(R.p * R.ir - L.p * L.ir) * rcp((float)(g - 1.0));
+}
+
 // Einfeldt speeds (the reference's HLL/HLLC estimate, include/scheme.h:765-790) relative
 // to the face: with the Roe velocity q~, s_L - q_L = (q~ - q_L) - d and
 // s_R - q_R = (q~ - q_R) + d, where q~ - q_L = sqrt(rho_R) dq / (sqrt(rho_L)+sqrt(rho_R))
@@ -363,49 +399,65 @@ __device__ __forceinline__ void hll(double g, const double cl[4], const double c
         F[m] += (double)(k * (w.aR * (float)(cr[m] - cl[m]) - dq * (float)cl[m] - dP[m]));
 }
 
-template <int DIR>
+// Roe; with HC RoeHC (include/scheme.h:2793-3110): eps = 1e-12 in the wave-strength
+// reciprocals and the energy flux less 0.5 |q~ + c~| (e_R - e_L) rho~.  Unlike the HLL
+// family, the wave algebra runs in FP64 (hardware reciprocal / square root with one
+// Newton step): the acoustic wave strengths grow like the Mach number
+// (a0, a3 ~ -/+ rho~ dn / (2 c~)) and largely cancel in the dissipation sum, and in a
+// cold hypersonic flow (Noh's inflow: internal energy ~1e-6 of the total) FP32 cannot
+// resolve the energy dissipation.  With FP32 wave algebra the mixed Noh state left the
+// FP64 one by 3e-5 (relative L1) within 10 steps; the jump forms of HLL / HLLC stay at
+// 1e-8.  The reconstruction stays FP32.
+template <int DIR, bool HC = false>
 __device__ __forceinline__ void roe(double g, const double cl[4], const double cr[4], double F[4]) {
     const Prim64 L = prim64(g, cl), R = prim64(g, cr);
     double FL[4], FR[4];
     phys64<DIR>(cl, L, FL);
     phys64<DIR>(cr, R, FR);
-    const float gm1 = (float)(g - 1.0), gf = (float)g;
-    const float sL = fsqrt((float)L.r), sR = fsqrt((float)R.r);
-    const float iss = rcp(sL + sR);
-    const float rbar = sL * sR;
-    const float uL = (float)L.u, uR = (float)R.u, vL = (float)L.v, vR = (float)R.v;
-    const float ubar = (sL * uL + sR * uR) * iss, vbar = (sL * vL + sR * vR) * iss;
-    const float aL2 = gf * (float)(L.p * L.ir), aR2 = gf * (float)(R.p * R.ir);
-    const float du = uR - uL, dv = vR - vL;
-    const float c2 = fmaxf((sL * aL2 + sR * aR2) * iss + 0.5f * gm1 * rbar * iss * iss * (du * du + dv * dv),
-                           1.0e-30f);
-    const float cbar = fsqrt(c2);
-    const float q2 = ubar * ubar + vbar * vbar;
-    const float Hbar = c2 * rcp(gm1) + 0.5f * q2;
-    const float nbar = DIR == 1 ? ubar : vbar;
-    const float lam0 = fabsf(nbar - cbar), lam1 = fabsf(nbar), lam3 = fabsf(nbar + cbar);
-    const float dr = (float)(cr[0] - cl[0]), dp = (float)(R.p - L.p);
-    const float i2cs = 0.5f * rcp(c2), ics = rcp(c2);
-    const float dn = DIR == 1 ? du : dv, dtg = DIR == 1 ? dv : du;
-    const float a0 = (dp - rbar * cbar * dn) * i2cs, a1 = dr - dp * ics, a2 = rbar * dtg,
-                a3 = (dp + rbar * cbar * dn) * i2cs;
-    const float t0 = 0.5f * lam0 * a0, t1 = 0.5f * lam1 * a1, t2 = 0.5f * lam1 * a2, t3 = 0.5f * lam3 * a3;
-    float D[4];
+    const double gm1 = g - 1.0;
+    const double sL = sqrt64(L.r), sR = sqrt64(R.r);
+    const double iss = rcp64(sL + sR);
+    const double rbar = sL * sR;
+    const double ubar = fma(sL, L.u, sR * R.u) * iss, vbar = fma(sL, L.v, sR * R.v) * iss;
+    const double aL2 = g * (L.p * L.ir), aR2 = g * (R.p * R.ir);
+    const double du = R.u - L.u, dv = R.v - L.v;
+    const double c2 = fmax(fma(fma(sL, aL2, sR * aR2), iss, (0.5 * gm1) * rbar * iss * iss *
+                                                               fma(du, du, dv * dv)),
+                           1.0e-300);
+    const double cbar = sqrt64(c2);
+    const double q2 = fma(ubar, ubar, vbar * vbar);
+    const double Hbar = fma(c2, rcp64(gm1), 0.5 * q2);
+    const double nbar = DIR == 1 ? ubar : vbar;
+    const double lam0 = fabs(nbar - cbar), lam1 = fabs(nbar), lam3 = fabs(nbar + cbar);
+    const double dr = cr[0] - cl[0], dp = R.p - L.p;
+    const double i2cs = HC ? rcp64(fma(2.0, c2, 1.0e-12)) : 0.5 * rcp64(c2);
+    const double ics = HC ? rcp64(c2 + 1.0e-12) : rcp64(c2);
+    const double dn = DIR == 1 ? du : dv, dtg = DIR == 1 ? dv : du;
+    const double rcdn = rbar * cbar * dn;
+    const double a0 = (dp - rcdn) * i2cs, a1 = fma(-dp, ics, dr), a2 = rbar * dtg,
+                 a3 = (dp + rcdn) * i2cs;
+    const double t0 = 0.5 * lam0 * a0, t1 = 0.5 * lam1 * a1, t2 = 0.5 * lam1 * a2,
+                 t3 = 0.5 * lam3 * a3;
+    const double s03 = t0 + t3, d03 = t3 - t0;  // the acoustic pair: sum and difference
+    double D[4];
+    D[0] = s03 + t1;
     if constexpr (DIR == 1) {
-        D[0] = t0 + t1 + t3;
-        D[1] = t0 * (ubar - cbar) + t1 * ubar + t3 * (ubar + cbar);
-        D[2] = (t0 + t1 + t3) * vbar + t2;
-        D[3] = t0 * (Hbar - ubar * cbar) + t1 * 0.5f * q2 + t2 * vbar + t3 * (Hbar + ubar * cbar);
+        D[1] = fma(ubar, D[0], cbar * d03);
+        D[2] = fma(vbar, D[0], t2);
+        D[3] = fma(Hbar, s03, fma(ubar * cbar, d03, fma(t1, 0.5 * q2, t2 * vbar)));
     } else {
-        D[0] = t0 + t1 + t3;
-        D[1] = (t0 + t1 + t3) * ubar + t2;
-        D[2] = t0 * (vbar - cbar) + t1 * vbar + t3 * (vbar + cbar);
-        D[3] = t0 * (Hbar - vbar * cbar) + t1 * 0.5f * q2 + t2 * ubar + t3 * (Hbar + vbar * cbar);
+        D[1] = fma(ubar, D[0], t2);
+        D[2] = fma(vbar, D[0], cbar * d03);
+        D[3] = fma(Hbar, s03, fma(vbar * cbar, d03, fma(t1, 0.5 * q2, t2 * ubar)));
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]) - (double)D[k];
+    for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]) - D[k];
+    if constexpr (HC) F[3] -= 0.5 * lam3 * ((R.p * R.ir - L.p * L.ir) * rcp64(gm1)) * rbar;
 }
 
+// LLF: the dissipation 0.5 s (U_R - U_L) multiplies the FP64 jumps (the FP32 speed s
+// only scales it; rounding the jumps themselves would, as for Roe, swamp the energy
+// dissipation of a cold hypersonic flow).
 template <int DIR>
 __device__ __forceinline__ void llf(double g, const double cl[4], const double cr[4], double F[4]) {
     const Prim64 L = prim64(g, cl), R = prim64(g, cr);
@@ -415,9 +467,84 @@ __device__ __forceinline__ void llf(double g, const double cl[4], const double c
     const float gf = (float)g;
     const float aL = fsqrt(fmaxf(gf * (float)(L.p * L.ir), 0.0f)), aR = fsqrt(fmaxf(gf * (float)(R.p * R.ir), 0.0f));
     const float qL = (float)(DIR == 1 ? L.u : L.v), qR = (float)(DIR == 1 ? R.u : R.v);
-    const float s = fmaxf(fabsf(qL) + aL, fabsf(qR) + aR);
+    const double hs = (double)(0.5f * fmaxf(fabsf(qL) + aL, fabsf(qR) + aR));
 #pragma unroll
-    for (int k = 0; k < 4; k++) F[k] = 0.5 * (FL[k] + FR[k]) - (double)(0.5f * s * (float)(cr[k] - cl[k]));
+    for (int k = 0; k < 4; k++) F[k] = fma(-hs, cr[k] - cl[k], 0.5 * (FL[k] + FR[k]));
+}
+
+// ------------------------------------------------- heat-capacity solvers (mixed)
+// HLLHC (include/scheme.h:1860-2137): the HLL flux of hll() and, between the waves, the
+// energy term (s_L s_R / (s_R - s_L)) (e_R - e_L) rho_HLL with the HLL density
+// rho_HLL = (s_R rho_R - s_L rho_L + F_L0 - F_R0) / (s_R - s_L), which in speeds relative
+// to each side is (rho_R (s_R - q_R) - rho_L (s_L - q_L)) / (s_R - s_L).  (The reference's
+// NaN fallback of the y-face wave speed cannot arise here: the mixed form hands the
+// solver admissible states only.)
+template <int DIR>
+__device__ __forceinline__ void hllhc(double g, const double cl[4], const double cr[4], double F[4]) {
+    const Prim64 L = prim64(g, cl), R = prim64(g, cr);
+    const Speeds<DIR> w = einfeldt<DIR>(g, L, R);
+    if (w.sl >= 0.0f) {
+        phys64<DIR>(cl, L, F);
+        return;
+    }
+    if (w.sr <= 0.0f) {
+        phys64<DIR>(cr, R, F);
+        return;
+    }
+    phys64<DIR>(cl, L, F);
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const float dq = (float)(qR - qL);
+    const float isd = rcp(w.sr - w.sl);
+    const float k = w.sl * isd;
+    float dP[4] = {0.0f, 0.0f, 0.0f, (float)(R.p * qR - L.p * qL)};
+    dP[DIR] = (float)(R.p - L.p);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+        F[m] += (double)(k * (w.aR * (float)(cr[m] - cl[m]) - dq * (float)cl[m] - dP[m]));
+    const float rhll = ((float)R.r * w.aR - (float)L.r * w.aL) * isd;
+    F[3] += (double)(k * w.sr * energy_jump(g, L, R) * rhll);
+}
+
+// HLLCHC (include/scheme.h:2153-2478): the HLLC flux of hllc() and, in either star region,
+// the energy term (s_L s_R / (s_R - s_L)) (e_R - e_L) rho*_K with the star density
+// rho*_K = rho_K (s_K - q_K) / (s_K - s*).
+template <int DIR>
+__device__ __forceinline__ void hllchc(double g, const double cl[4], const double cr[4], double F[4]) {
+    const Prim64 L = prim64(g, cl), R = prim64(g, cr);
+    const Speeds<DIR> w = einfeldt<DIR>(g, L, R);
+    if (w.sl >= 0.0f) {
+        phys64<DIR>(cl, L, F);
+        return;
+    }
+    if (w.sr <= 0.0f) {
+        phys64<DIR>(cr, R, F);
+        return;
+    }
+    const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
+    const float rL = (float)L.r, rR = (float)R.r;
+    const float dq = (float)(qR - qL), dp = (float)(R.p - L.p);
+    const float mL = rL * w.aL, mR = rR * w.aR;
+    const float im = rcp(mL - mR);
+    const float sqstL = (dp - mR * dq) * im, sqstR = (dp - mL * dq) * im;
+    const float sstar = (float)qL + sqstL;
+    const bool left = sstar >= 0.0f;
+    const double* c = left ? cl : cr;
+    const Prim64& S = left ? L : R;
+    const float s = left ? w.sl : w.sr, r = left ? rL : rR;
+    const float sq = left ? w.aL : w.aR;
+    const float sqst = left ? sqstL : sqstR;
+    phys64<DIR>(c, S, F);
+    const float iden = rcp(sq - sqst);  // 1 / (s - s*)
+    const float chi = sqst * iden;
+    const float sc = s * chi;
+    const float tang = (float)(DIR == 1 ? S.v : S.u);
+    const float dE = (float)(c[3] + S.p) + r * sq * sstar;
+    const float ec = w.sl * w.sr * rcp(w.sr - w.sl) * energy_jump(g, L, R);
+    F[0] += (double)(sc * r);
+    const double dn = (double)(sc * r * s), dt = (double)(sc * r * tang);
+    F[1] += DIR == 1 ? dn : dt;
+    F[2] += DIR == 1 ? dt : dn;
+    F[3] += (double)(fmaf(sc, dE, ec * (r * sq * iden)));
 }
 
 template <int DIR, int SOLVER>
@@ -425,6 +552,9 @@ __device__ __forceinline__ void riemann(double g, const double cl[4], const doub
     if constexpr (SOLVER == SOLVER_HLL) hll<DIR>(g, cl, cr, F);
     else if constexpr (SOLVER == SOLVER_HLLC) hllc<DIR>(g, cl, cr, F);
     else if constexpr (SOLVER == SOLVER_ROE) roe<DIR>(g, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLHC) hllhc<DIR>(g, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_HLLCHC) hllchc<DIR>(g, cl, cr, F);
+    else if constexpr (SOLVER == SOLVER_ROEHC) roe<DIR, true>(g, cl, cr, F);
     else llf<DIR>(g, cl, cr, F);
 }
 

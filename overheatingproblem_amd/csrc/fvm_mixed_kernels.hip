@@ -33,8 +33,18 @@ __device__ __forceinline__ int face_fp64(const Phys& ph, const double st[6][4], 
 }
 
 // BUF: loads and stores through buffer descriptors (fvm_buffer.hpp), for arrays below 4 GB.
+// Minimum waves/SIMD of the x sweep (0: the compiler's choice; WENO5-char: 111 VGPRs,
+// 4 waves; a bound of 5 holds 96 VGPRs and spills 17 dwords).
+#ifndef RPFVM_FLUX_MX_WAVES
+#define RPFVM_FLUX_MX_WAVES 0
+#endif
+#if RPFVM_FLUX_MX_WAVES > 0
+#define FLUX_MX_BOUNDS __launch_bounds__(256, RPFVM_FLUX_MX_WAVES)
+#else
+#define FLUX_MX_BOUNDS __launch_bounds__(256)
+#endif
 template <int DIR, int ORDER, bool CHAR, int SOLVER, bool BUF>
-__global__ FLUX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
+__global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
                                                     double* __restrict__ F, Geom g, Phys ph,
                                                     FaceFix fix) {
     const int j0 = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64;
@@ -273,6 +283,9 @@ static FluxPair mpick2(int solver) {
         case SOLVER_HLL: return mpick3<ORDER, CHAR, SOLVER_HLL>();
         case SOLVER_HLLC: return mpick3<ORDER, CHAR, SOLVER_HLLC>();
         case SOLVER_ROE: return mpick3<ORDER, CHAR, SOLVER_ROE>();
+        case SOLVER_HLLHC: return mpick3<ORDER, CHAR, SOLVER_HLLHC>();
+        case SOLVER_HLLCHC: return mpick3<ORDER, CHAR, SOLVER_HLLCHC>();
+        case SOLVER_ROEHC: return mpick3<ORDER, CHAR, SOLVER_ROEHC>();
         default: return mpick3<ORDER, CHAR, SOLVER_LLF>();
     }
 }

@@ -7,6 +7,8 @@
     python -m overheatingproblem_amd.profiling accuracy <noh_full.json> <name> "<command>"
     python -m overheatingproblem_amd.profiling solvers <noh_solvers.json> <name> "<command>"
     python -m overheatingproblem_amd.profiling ll3     <ll3_full.json> <name> "<command>"
+    python -m overheatingproblem_amd.profiling dmr     <dmr_full.json> <name> "<command>"
+    python -m overheatingproblem_amd.profiling c1      <c1_full.json> <name> "<command>"
 
 `trace` turns a `rocprofv3 --kernel-trace --stats --output-format csv` directory into a
 markdown table of per-kernel calls and average durations.  `traffic` turns two
@@ -132,37 +134,114 @@ def valu_markdown(v: dict, name: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def _region_row(label: str, r: dict, regions) -> str:
+    cells = " | ".join(f"{r['l1_' + k]:.2e} ({r['share_' + k]:.2f})" for k in regions)
+    at = f" at {tuple(r['linf_at'])}" if "linf_at" in r else ""
+    return f"| {label} | {r['l1']:.3e} | {cells} | {r['linf']:.2f}{at} |"
+
+
+def _history_table(d: dict, runs, regions) -> list:
+    head = "| step | t | " + " | ".join(f"{lab} L1 | {lab} L1 rest" for lab, _ in runs) + " |"
+    lines = [head, "|" + "---:|" * (2 + 2 * len(runs))]
+    for h in d["history"]:
+        cells = " | ".join(f"{h[k]['l1']:.2e} | {h[k]['l1_rest']:.2e}" for _, k in runs)
+        lines.append(f"| {h['step']} | {h['t']:.4f} | {cells} |")
+    return lines
+
+
 def accuracy_markdown(d: dict, command: str) -> str:
     """The Noh full-length accuracy record from `bench.py --noh-full`'s JSON line."""
     runs = (("FP64", "fp64"), ("mixed (own dts)", "mixed"), ("mixed (FP64 dts)", "mixed_replay"),
-            ("FP64, rho perturbed 1e-12", "fp64_perturbed"))
+            ("FP64, rho perturbed 1e-12 (FP64 dts)", "fp64_perturbed"))
+    regions = ("corner", "axes", "shock", "rest")
     n = d["workload"].split("-")[1]
     lines = [f"# Noh {n} to t = {d['t_end']}: accuracy of the mixed form", "",
              "Command (from the repository root):", "", "```", command, "```", "",
              "Noh implosion, gamma 5/3, p0 = 1e-6, WENO5-char + HLLC, SSP-RK3, CFL 0.4, "
              "positivity safeguard on, one MI355X.  Exact solution: rho = 16 behind the shock "
-             "(r < t/3); the wall-heating deficit is 1 - rho/16 near the origin.", "",
-             "| run | steps | wall s | rho(corner cell) | deficit corner | deficit r<0.05 | "
+             "(r < t/3); the wall-heating deficit is 1 - rho/16 near the origin.  FP64 runs the "
+             "driver loop; the mixed form and FP64 from a density perturbed by 1e-12 "
+             "(relative) run on its dts in lockstep, compared with it every "
+             f"{d['every']} steps; then the mixed form once more on its own dts.", "",
+             "| run | steps | faults | rho(corner cell) | deficit corner | deficit r<0.05 | "
              "rho plateau 0.1<r<0.15 |", "|---|---:|---:|---:|---:|---:|---:|"]
     for label, k in runs:
         r = d[k]
-        lines.append(f"| {label} | {r['steps']} | {r['wall_s']:.2f} | {r['rho_corner']:.4f} | "
+        lines.append(f"| {label} | {r['steps']} | {r['faults']} | {r['rho_corner']:.4f} | "
                      f"{r['deficit_corner']:+.4f} | {r['deficit_r_lt_0.05']:+.5f} | "
                      f"{r['rho_plateau_0.1_0.15']:.4f} |")
-    lines += ["", "Differences from the FP64 run (interior, absolute and relative to mean |FP64|):",
-              "", "| compared with FP64 | L1(rho) | rel | Linf(rho) | L1(p) | rel | Linf(p) |",
-              "|---|---:|---:|---:|---:|---:|---:|"]
+    lines += ["", f"Density difference from the FP64 run at t = {d['t_end']}, by region "
+              f"({d['regions']}): L1 = mean |difference| over the region's cells, with the "
+              "region's share of the whole grid's sum in parentheses; L-inf and its interior "
+              "cell (i, j).", "",
+              "| compared with FP64 | L1 | " + " | ".join(f"L1 {k}" for k in regions) +
+              " | L-inf |", "|---|---:|" + "---:|" * len(regions) + "---|"]
     for label, k in runs[1:]:
-        c = d[k + "_vs_fp64"]
-        lines.append(f"| {label} | {c['l1_rho']:.3e} | {c['l1_rho_rel']:.3e} | {c['linf_rho']:.3f} "
-                     f"| {c['l1_p']:.3e} | {c['l1_p_rel']:.3e} | {c['linf_p']:.3f} |")
-    ref = d["fp64_perturbed_vs_fp64"]["l1_rho"]
-    ratios = ", ".join(f"{label} {d[k + '_vs_fp64']['l1_rho'] / ref:.2f}" for label, k in runs[1:3])
-    lines += ["", "Reading: the Noh solution amplifies round-off at the origin (wall heating) and at "
-              "the shock, so over the full run FP64 itself, started from a density perturbed by "
-              "1e-12, ends this far from the unperturbed FP64 run; the mixed form's L1(rho) "
-              f"distance from FP64, in units of that FP64 self-distance: {ratios}."]
+        lines.append(_region_row(label, d["final_regions"][k], regions))
+    lines += ["", "History (FP64 dts, lockstep): L1 over the whole grid and over the rest "
+              "(corner, wall bands and shock band masked).", ""]
+    lines += _history_table(d, (("mixed", "mixed_replay"), ("perturbed FP64", "fp64_perturbed")), regions)
+    fr = d["final_regions"]
+    ratio = fr["mixed_replay"]["l1_rest"] / fr["fp64_perturbed"]["l1_rest"]
+    lines += ["", "Reading: the masked L1 (rest) of the mixed form is "
+              f"{fr['mixed_replay']['l1_rest']:.2e} on the FP64 dts, {ratio:.2f} x the distance "
+              "FP64 itself travels from a 1e-12 density perturbation "
+              f"({fr['fp64_perturbed']['l1_rest']:.2e}); both grow from the origin corner "
+              "outwards along the walls and into the post-shock plateau (history)."]
     return "\n".join(lines) + "\n"
+
+
+def dmr_markdown(d: dict, command: str) -> str:
+    """The double-Mach-reflection full-length record from `bench.py --dmr-full`."""
+    regions = ("fronts", "rest")
+    lines = [f"# Double Mach reflection {d['workload'].split('-')[1]} to t = {d['t_end']}: "
+             "accuracy and throughput of the mixed form", "",
+             "Command (from the repository root):", "", "```", command, "```", "",
+             "Mach-10 double Mach reflection, moving boundaries, WENO5-char + Roe with the "
+             "positivity safeguard (HLLE fallback), SSP-RK3, CFL 0.5, one MI355X.  FP64 runs the "
+             "driver loop; the mixed form and FP64 from a density perturbed by 1e-12 (relative) "
+             f"run on its dts in lockstep, compared every {d['every']} steps; then the mixed "
+             f"form on its own dts.  Regions: {d['regions']} ({100 * d['fronts_fraction']:.2f}% "
+             "of the cells are fronts).", "",
+             "| run | steps | faults | Mcell-updates/s |", "|---|---:|---:|---:|",
+             f"| FP64 (driver loop) | {d['fp64']['steps']} | {d['fp64']['faults']} | |",
+             f"| mixed (own dts) | {d['mixed']['steps']} | {d['mixed']['faults']} | "
+             f"{d['mixed']['mcell_updates_per_s']:.0f} |", "",
+             f"Density difference from the FP64 run at t = {d['t_end']} (L1 by region, share in "
+             "parentheses), and L1 / L-inf of pressure:", "",
+             "| compared with FP64 | L1(rho) | L1 fronts | L1 rest | L-inf(rho) | L1(p) | "
+             "L-inf(p) |", "|---|---:|---:|---:|---|---:|---:|"]
+    for label, k in (("mixed (FP64 dts)", "mixed_replay"), ("FP64, rho perturbed 1e-12", "fp64_perturbed"),
+                     ("mixed (own dts)", "mixed")):
+        r, pp = d["final_regions"][k], d["final_p"][k]
+        lines.append(f"| {label} | {r['l1']:.3e} | {r['l1_fronts']:.2e} ({r['share_fronts']:.2f}) | "
+                     f"{r['l1_rest']:.2e} ({r['share_rest']:.2f}) | {r['linf']:.2f} at "
+                     f"{tuple(r['linf_at'])} | {pp['l1']:.3e} | {pp['linf']:.2f} |")
+    lines += ["", "History (FP64 dts, lockstep):", ""]
+    lines += _history_table(d, (("mixed", "mixed_replay"), ("perturbed FP64", "fp64_perturbed")), regions)
+    return "\n".join(lines) + "\n"
+
+
+def c1_markdown(d: dict, command: str) -> str:
+    """The C1 record (Sedov 200^2 to t = 1.0) from `bench.py --c1-full`."""
+    c, g, b = d["cpu_reference"], d["gpu_fp64"], d["bit_identical"]
+    h = c["host"]
+    return "\n".join([
+        "# Sedov-type blast 200 x 200 to t = 1.0: the reference's CPU step and the FP64 form", "",
+        "Command (from the repository root):", "", "```", command, "```", "",
+        "Sedov-type blast on [-1,1]^2, gamma 1.4, rho = 1, p = 1e-5, E0 = 1 in the cells with "
+        "r <= 2 dx; WENO5-char + HLLC, SSP-RK3, CFL 0.5, outflow, t_end = 1.0.  The reference's "
+        "own step (oracle/_ref/ref_harness, mode run: the driver loop of src/RP_FVM.c:179-191, "
+        "dt + step timed as src/RP_FVM.c:181-216, IO excluded) on the GPU host's cores, then the "
+        "FP64 form on the GPU through the same loop (one dt read back per step).", "",
+        "| run | steps | seconds | Mcell-updates/s | cores |", "|---|---:|---:|---:|---:|",
+        f"| reference CPU/OpenMP step | {d['steps']} | {c['seconds']:.2f} | {c['value']:.3f} | "
+        f"{c['cores']} ({h['cpu_model']}, {h['cpus_visible']} CPUs visible, OMP_NUM_THREADS="
+        f"{h['omp_num_threads_env']}) |",
+        f"| FP64 form, one MI355X | {g['steps']} | {g['seconds_wall']:.2f} | {g['value']:.1f} | |", "",
+        f"Bit-identical to the reference: dts {b['dts']}, final state {b['state']}, final time "
+        f"{b['t']}.  At 200^2 the GPU loop is bound by one launch sequence and one host "
+        "round trip per step, not by the step's arithmetic."]) + "\n"
 
 
 def ll3_markdown(d: dict, command: str) -> str:
@@ -187,25 +266,28 @@ def ll3_markdown(d: dict, command: str) -> str:
 def solvers_markdown(d: dict, command: str) -> str:
     """The per-solver Noh wall-heating record from `bench.py --noh-solvers`'s JSON line."""
     n = d["workload"].split("-")[1]
-    lines = [f"# Noh {n} to t = {d['t_end']}: wall heating per Riemann solver (FP64 form)", "",
+    lines = [f"# Noh {n} to t = {d['t_end']}: wall heating per Riemann solver and form", "",
              "Command (from the repository root):", "", "```", command, "```", "",
              "Noh implosion, gamma 5/3, p0 = 1e-6, WENO5-char, SSP-RK3, CFL 0.4, positivity "
              "safeguard on, one MI355X.  Exact solution: rho = 16 behind the shock (r < t/3); "
              "the wall-heating deficit is 1 - rho/16 at the origin (corner cell) and over "
              "r < 0.05.  Fault bits (RPFVM_FAULT_*): 1 = an HLL face or RoeHC face took the "
-             "reference's bad-state skip, 2 = non-finite dt, 8 = the safeguard left a cell with "
-             "rho <= 0 or p <= 0.", "",
-             "| solver | scheme | steps | t | faults | rho(corner cell) | deficit corner | "
-             "deficit r<0.05 | rho plateau 0.1<r<0.15 | Mcell-updates/s |",
-             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+             "reference's bad-state skip, 2 = non-finite dt, 4 = the mixed form recomputed "
+             "faces in FP64 (informational), 8 = the safeguard left a cell with rho <= 0 or "
+             "p <= 0.  Each form runs on its own dts; L1 is the mixed run's density distance "
+             "from the FP64 run of the same solver.", "",
+             "| solver | scheme | form | steps | faults | rho(corner cell) | deficit corner | "
+             "deficit r<0.05 | rho plateau 0.1<r<0.15 | L1(rho) vs FP64 | Mcell-updates/s |",
+             "|---|---:|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
     for r in d["solvers"]:
         if r["finite"]:
             m = (f"{r['rho_corner']:.4f} | {r['deficit_corner']:+.4f} | "
                  f"{r['deficit_r_lt_0.05']:+.5f} | {r['rho_plateau_0.1_0.15']:.4f}")
         else:
             m = "non-finite | | |"
-        lines.append(f"| {r['solver']} | {r['scheme']} | {r['steps']} | {r['t']:.4f} | {r['faults']} "
-                     f"| {m} | {r['mcell_updates_per_s']:.0f} |")
+        l1 = f"{r['l1_rho_vs_fp64']:.2e}" if "l1_rho_vs_fp64" in r else ""
+        lines.append(f"| {r['solver']} | {r['scheme']} | {r.get('form', 'fp64')} | {r['steps']} "
+                     f"| {r['faults']} | {m} | {l1} | {r['mcell_updates_per_s']:.0f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -254,6 +336,15 @@ def main(argv):
         path = os.path.join(PROFILES, f"{name}_accuracy.md")
         with open(path, "w") as fh:
             fh.write(accuracy_markdown(d, command))
+        print(path)
+        return 0
+    if len(argv) >= 4 and argv[0] in ("dmr", "c1"):
+        kind, jpath, name, command = argv[:4]
+        with open(jpath) as fh:
+            d = json.loads(fh.read().strip().splitlines()[-1])
+        path = os.path.join(PROFILES, f"{name}_{'accuracy' if kind == 'dmr' else 'record'}.md")
+        with open(path, "w") as fh:
+            fh.write(dmr_markdown(d, command) if kind == "dmr" else c1_markdown(d, command))
         print(path)
         return 0
     if len(argv) >= 4 and argv[0] == "ll3":

@@ -43,7 +43,8 @@ def rel_l1(a, b):
 
 
 @pytest.mark.parametrize("scheme,recon,char", [(2, 5, 1), (3, 5, 0), (1, 2, 1), (10, 3, 1),
-                                               (2, 1, 0), (3, 3, 0)])
+                                               (2, 1, 0), (3, 3, 0), (11, 5, 1), (22, 5, 1),
+                                               (33, 5, 0), (22, 2, 0)])
 def test_fp64_recompute_route_is_exact(scheme, recon, char):
     """With every face sent through the mixed form's FP64 recompute list, the result is
     the FP64 path's, bit for bit (tests the list/fix-up machinery)."""
@@ -55,7 +56,7 @@ def test_fp64_recompute_route_is_exact(scheme, recon, char):
     assert np.array_equal(U64, Umx)
 
 
-@pytest.mark.parametrize("scheme", [2, 1, 3, 10])
+@pytest.mark.parametrize("scheme", [2, 1, 3, 10, 11, 22, 33])
 def test_mixed_close_to_fp64_lax_liu(scheme):
     c, U0 = cases.lax_liu_3(256, scheme=scheme)
     U64, dts, f64 = run(c, U0, steps=60)
@@ -104,10 +105,12 @@ def test_mixed_smooth_wave_accuracy():
     assert abs(emx / e64 - 1.0) < 1e-2, (e64, emx)
 
 
-def test_mixed_noh_positivity():
-    """Noh (p = 1e-6) with the safeguard: the mixed form stays admissible and within
-    L1(rho) <= 1e-5 (relative) of the FP64 form over 150 steps."""
-    c, U0 = cases.noh(256, positivity=True)
+@pytest.mark.parametrize("scheme", [2, 11, 22, 33])
+def test_mixed_noh_positivity(scheme):
+    """Noh (p = 1e-6) with the safeguard, HLLC and the heat-capacity solvers (the wall
+    heating they address): the mixed form stays admissible and within L1(rho) <= 1e-5
+    (relative) of the FP64 form over 150 steps."""
+    c, U0 = cases.noh(256, positivity=True, scheme=scheme)
     U64, dts, f64 = run(c, U0, steps=150)
     Umx, _, fmx = run(mixed(c), U0, dts=dts)
     assert (f64 & 8) == 0 and (fmx & 8) == 0
