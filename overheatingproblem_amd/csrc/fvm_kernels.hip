@@ -78,12 +78,18 @@ __global__ FLUX_BOUNDS void k_flux(const double* __restrict__ Y, double* __restr
 #ifndef RPFVM_RING_WAVES
 #define RPFVM_RING_WAVES 1
 #endif
+#ifndef RPFVM_RING_OCC
+#define RPFVM_RING_OCC 3
+#endif
+#ifndef RPFVM_MARCH_OCC
+#define RPFVM_MARCH_OCC 3
+#endif
 constexpr int kRingRows = RPFVM_RING_ROWS;    // face rows per wavefront
 constexpr int kRingWaves = RPFVM_RING_WAVES;  // wavefronts per workgroup
 constexpr int kRingLds = 6 * 4 * 64;          // doubles per wavefront (12 KB: 3 waves/SIMD)
 
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(64 * kRingWaves, 3) void k_flux_ring(const double* __restrict__ Y,
+__global__ __launch_bounds__(64 * kRingWaves, RPFVM_RING_OCC) void k_flux_ring(const double* __restrict__ Y,
                                                                  double* __restrict__ F, Geom g,
                                                                  Phys ph, unsigned* __restrict__ faults) {
     __shared__ double lds[kRingWaves * kRingLds];
@@ -207,10 +213,11 @@ __global__ FUSED_BOUNDS void k_yflux_update(const double* Un, const double* Y,
 
 // The marching form (yflux_update_march) of k_yflux_update: identical results, 8% faster
 // at 4096^2 (WENO5-char HLLC); used whenever an array's 4 planes fit in 4 GB.  Bounded
-// to 3 waves/SIMD like the row kernel (WENO5-char: 167 VGPRs and 6 spilled; a 4-wave
-// bound spills 61).
+// to 3 waves/SIMD like k_flux_ring (WENO5-char: 167 VGPRs, no spill).  A 4-wave bound
+// (RPFVM_MARCH_OCC / RPFVM_RING_OCC = 4) spills 120 and 84 bytes a lane and makes the
+// WENO5-char HLLC step at 4096^2 17% slower.
 template <int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(64 * kMarchWaves, 3) void k_yflux_march(
+__global__ __launch_bounds__(64 * kMarchWaves, RPFVM_MARCH_OCC) void k_yflux_march(
     const double* Un, const double* Y, double* out, const double* __restrict__ F, double* G, Geom g,
     Phys ph, double dt, int kind, Repair rp, DtSlots ds, unsigned* __restrict__ faults) {
     __shared__ double lds[kMarchWaves * kMarchLds];

@@ -51,6 +51,32 @@ __device__ __forceinline__ double quot(double a, const Recip& r) {
     return a / r.b;
 }
 
+// Quotients in groups.  The per-quotient test of quot() costs each quotient a divergent
+// branch (an exec-mask save, a skip and a restore: about as many scalar instructions as
+// the quotient has vector ones).  A group of quotients (one WENO weight combination, the
+// primitives of a cell, the Roe-averaged basis of a face) instead runs the fast path of
+// every quotient unconditionally with Div<true>, which records whether each was inside
+// the exact range; one vote of the wavefront then keeps the group, or reruns it with
+// Div<false> (quot() per quotient) when any lane left the range.  Either way every lane
+// gets exactly the IEEE quotients.
+template <bool FAST>
+struct Div {
+    bool ok = true;
+    __device__ __forceinline__ double operator()(double a, const Recip& r) {
+        if constexpr (FAST) {
+            ok = ok && r.ok && in_safe_range(a);
+            const double q = a * r.y;
+            const double rem = fma(-r.b, q, a);
+            return fma(rem, r.y, q);
+        } else {
+            return quot(a, r);
+        }
+    }
+};
+
+// True when `ok` holds in every active lane of the wavefront (a uniform value).
+__device__ __forceinline__ bool all_lanes(bool ok) { return __ballot(!ok) == 0ull; }
+
 // ------------------------------------------------------------------- helpers
 // sgn / max_of_two / min_of_two / min_mod / van_leer: include/function.h:31-74
 __device__ __forceinline__ int sgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
@@ -83,16 +109,27 @@ __device__ __forceinline__ Beta5 beta5(double v1, double v2, double v3, double v
 // Weighted combination of the three third-order candidates, :1567-1569.  r6 is the
 // Recip of 6.0 (built from a runtime value so the compiler cannot fold the hardware
 // reciprocal estimate).
+template <bool FAST>
+__device__ __forceinline__ double weno5_combine_t(double a1, double a2, double a3, double v1,
+                                                  double v2, double v3, double v4, double v5,
+                                                  const Recip& r6, Div<FAST>& q) {
+    const Recip rs = recip(a1 + a2 + a3);
+    const double w1 = q(a1, rs);
+    const double w2 = q(a2, rs);
+    const double w3 = q(a3, rs);
+    return q(w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3), r6)
+         + q(w2 * (-v2 + 5.0 * v3 + 2.0 * v4), r6)
+         + q(w3 * (2.0 * v3 + 5.0 * v4 - v5), r6);
+}
+
 __device__ __forceinline__ double weno5_combine(double a1, double a2, double a3, double v1,
                                                 double v2, double v3, double v4, double v5,
                                                 const Recip& r6) {
-    const Recip rs = recip(a1 + a2 + a3);
-    const double w1 = quot(a1, rs);
-    const double w2 = quot(a2, rs);
-    const double w3 = quot(a3, rs);
-    return quot(w1 * (2.0 * v1 - 7.0 * v2 + 11.0 * v3), r6)
-         + quot(w2 * (-v2 + 5.0 * v3 + 2.0 * v4), r6)
-         + quot(w3 * (2.0 * v3 + 5.0 * v4 - v5), r6);
+    Div<true> f;
+    const double r = weno5_combine_t(a1, a2, a3, v1, v2, v3, v4, v5, r6, f);
+    if (all_lanes(f.ok)) return r;
+    Div<false> c;
+    return weno5_combine_t(a1, a2, a3, v1, v2, v3, v4, v5, r6, c);
 }
 
 // WENO5-Z (Borges et al. 2008, exponent 1, eps 1e-15): WENO5Z_L/R,
@@ -114,9 +151,15 @@ __device__ __forceinline__ double weno5js(double v1, double v2, double v3, doubl
     const double eps = 1.0e-15;
     const Beta5 b = beta5(v1, v2, v3, v4, v5);
     const Recip r1 = recip(eps + b.b1), r2 = recip(eps + b.b2), r3 = recip(eps + b.b3);
-    const double a1 = quot(quot(0.1, r1), r1);
-    const double a2 = quot(quot(0.6, r2), r2);
-    const double a3 = quot(quot(0.3, r3), r3);
+    Div<true> f;
+    double a1 = f(f(0.1, r1), r1);
+    double a2 = f(f(0.6, r2), r2);
+    double a3 = f(f(0.3, r3), r3);
+    if (!all_lanes(f.ok)) {
+        a1 = quot(quot(0.1, r1), r1);
+        a2 = quot(quot(0.6, r2), r2);
+        a3 = quot(quot(0.3, r3), r3);
+    }
     return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5, r6);
 }
 
@@ -184,112 +227,165 @@ __device__ __forceinline__ Consts make_consts(const Phys& ph) {
 // Con_to_Pri_2D for one cell (include/initialize.h:1443-1451) and the enthalpy
 // Compute_Eigen_2DX/2DY derive from it (include/Characteriz.h:318,420); the three
 // quotients by rho share one reciprocal.
+template <bool FAST>
 __device__ __forceinline__ void primitive_h(const Consts& k, const double c[4], double p[4],
-                                            double& H, Recip& rr) {
+                                            double& H, Recip& rr, Div<FAST>& q) {
     rr = recip(c[0]);
     const double rho = c[0];
-    const double u = quot(c[1], rr);
-    const double v = quot(c[2], rr);
+    const double u = q(c[1], rr);
+    const double v = q(c[2], rr);
     p[0] = rho;
     p[1] = u;
     p[2] = v;
     p[3] = (c[3] - 0.5 * rho * (u * u + v * v)) * k.gm1;
-    H = 0.5 * (u * u + v * v) + k.gr * quot(p[3], rr);
+    H = 0.5 * (u * u + v * v) + k.gr * q(p[3], rr);
 }
 
 // The left (EL[wave][var]) and right (ER[var][wave]) eigenvectors of the face between
 // cells a and b (conserved states).  DIR 1: Compute_Eigen_2DX (include/Characteriz.h:
 // 300-398); DIR 2: Compute_Eigen_2DY (:402-500), with its literal 0.0/c and -0.0 terms
 // (0.0/c == 0.0 * (1/c) for every c = sqrt(c2) >= +0 or NaN).
-template <int DIR>
-__device__ __forceinline__ void face_eigen(const Consts& k, const double ca[4], const double cb[4],
-                                           double EL[4][4], double ER[4][4]) {
+//
+// The 32 entries are simple products of nine face scalars, so a face keeps the scalars
+// (FaceBasis) and forms each row where it is used: a row of EL per projected wave, a row
+// of ER per back-projected variable.  Every entry is the same expression as in the
+// reference and is formed once per face, as when the matrices are held; holding nine
+// scalars instead of 32 entries is what keeps the characteristic kernels below 128
+// VGPRs.
+struct FaceBasis {
+    double u, v, H, q2, c, b1, b2, ic, wc;  // wc = u/c (DIR 1) or v/c (DIR 2)
+};
+
+template <int DIR, bool FAST>
+__device__ __forceinline__ FaceBasis face_basis_t(const Consts& k, const double ca[4],
+                                                  const double cb[4], Div<FAST>& quot) {
     double a[4], b[4], Ha, Hb;
     Recip ra, rb;
-    primitive_h(k, ca, a, Ha, ra);
-    primitive_h(k, cb, b, Hb, rb);
+    primitive_h(k, ca, a, Ha, ra, quot);
+    primitive_h(k, cb, b, Hb, rb, quot);
     const double D = sqrt(quot(b[0], ra));
     const double D1 = D + 1.0;
     const Recip rd = recip(D1);
-    const double u = quot(a[1] + D * b[1], rd);
-    const double v = quot(a[2] + D * b[2], rd);
-    const double H = quot(Ha + D * Hb, rd);
-    const double q2 = u * u + v * v;
-    const double c2 = k.gm1 * (H - 0.5 * q2);
-    const double c = sqrt(c2);
-    const double b1 = k.gm1 / c2;
-    const double b2 = 1.0 + b1 * q2 - b1 * H;
-    const Recip rc = recip(c);
-    const double ic = quot(1.0, rc);
+    FaceBasis e;
+    e.u = quot(a[1] + D * b[1], rd);
+    e.v = quot(a[2] + D * b[2], rd);
+    e.H = quot(Ha + D * Hb, rd);
+    e.q2 = e.u * e.u + e.v * e.v;
+    const double c2 = k.gm1 * (e.H - 0.5 * e.q2);
+    e.c = sqrt(c2);
+    e.b1 = k.gm1 / c2;
+    e.b2 = 1.0 + e.b1 * e.q2 - e.b1 * e.H;
+    const Recip rc = recip(e.c);
+    e.ic = quot(1.0, rc);
+    e.wc = quot(DIR == 1 ? e.u : e.v, rc);
+    return e;
+}
+
+template <int DIR>
+__device__ __forceinline__ FaceBasis face_basis(const Consts& k, const double ca[4],
+                                                const double cb[4]) {
+    Div<true> f;
+    const FaceBasis e = face_basis_t<DIR>(k, ca, cb, f);
+    if (all_lanes(f.ok)) return e;
+    Div<false> c;
+    return face_basis_t<DIR>(k, ca, cb, c);
+}
+
+// Row `w` of EL (the left eigenvector of wave w); w is a constant after unrolling.
+template <int DIR>
+__device__ __forceinline__ void el_row(const FaceBasis& e, int w, double r[4]) {
+    const double u = e.u, v = e.v, b1 = e.b1, b2 = e.b2, ic = e.ic, wc = e.wc;
     if constexpr (DIR == 1) {
-        const double uc = quot(u, rc);
-        EL[0][0] = 0.5 * (b2 + uc);
-        EL[0][1] = -0.5 * (b1 * u + ic);
-        EL[0][2] = -0.5 * b1 * v;
-        EL[0][3] = 0.5 * b1;
-        EL[1][0] = -q2 + H;
-        EL[1][1] = u;
-        EL[1][2] = v;
-        EL[1][3] = -1.0;
-        EL[2][0] = v;
-        EL[2][1] = 0.0;
-        EL[2][2] = -1.0;
-        EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - uc);
-        EL[3][1] = 0.5 * (-b1 * u + ic);
-        EL[3][2] = 0.5 * -b1 * v;
-        EL[3][3] = 0.5 * b1;
-        ER[0][0] = 1.0;
-        ER[0][1] = b1;
-        ER[0][2] = 0.0;
-        ER[0][3] = 1.0;
-        ER[1][0] = u - 1.0 * c;
-        ER[1][1] = u * b1;
-        ER[1][2] = 0.0;
-        ER[1][3] = u + 1.0 * c;
-        ER[2][0] = v - 0.0 * c;
-        ER[2][1] = v * b1;
-        ER[2][2] = -1.0;
-        ER[2][3] = v + 0.0 * c;
-        ER[3][0] = H - u * c;
-        ER[3][1] = H * b1 - 1.0;
-        ER[3][2] = -v;
-        ER[3][3] = H + u * c;
+        if (w == 0) {
+            r[0] = 0.5 * (b2 + wc);
+            r[1] = -0.5 * (b1 * u + ic);
+            r[2] = -0.5 * b1 * v;
+            r[3] = 0.5 * b1;
+        } else if (w == 1) {
+            r[0] = -e.q2 + e.H;
+            r[1] = u;
+            r[2] = v;
+            r[3] = -1.0;
+        } else if (w == 2) {
+            r[0] = v;
+            r[1] = 0.0;
+            r[2] = -1.0;
+            r[3] = 0.0;
+        } else {
+            r[0] = 0.5 * (b2 - wc);
+            r[1] = 0.5 * (-b1 * u + ic);
+            r[2] = 0.5 * -b1 * v;
+            r[3] = 0.5 * b1;
+        }
     } else {
-        const double vc = quot(v, rc);
         const double zc = 0.0 * ic;
-        EL[0][0] = 0.5 * (b2 + vc);
-        EL[0][1] = -0.5 * (b1 * u + zc);
-        EL[0][2] = -0.5 * (b1 * v + ic);
-        EL[0][3] = 0.5 * b1;
-        EL[1][0] = -q2 + H;
-        EL[1][1] = u;
-        EL[1][2] = v;
-        EL[1][3] = -1.0;
-        EL[2][0] = -u;
-        EL[2][1] = 1.0;
-        EL[2][2] = -0.0;
-        EL[2][3] = 0.0;
-        EL[3][0] = 0.5 * (b2 - vc);
-        EL[3][1] = 0.5 * (-b1 * u + zc);
-        EL[3][2] = 0.5 * (-b1 * v + ic);
-        EL[3][3] = 0.5 * b1;
-        ER[0][0] = 1.0;
-        ER[0][1] = b1;
-        ER[0][2] = 0.0;
-        ER[0][3] = 1.0;
-        ER[1][0] = u - 0.0 * c;
-        ER[1][1] = u * b1;
-        ER[1][2] = 1.0;
-        ER[1][3] = u + 0.0 * c;
-        ER[2][0] = v - 1.0 * c;
-        ER[2][1] = v * b1;
-        ER[2][2] = -0.0;
-        ER[2][3] = v + 1.0 * c;
-        ER[3][0] = H - v * c;
-        ER[3][1] = H * b1 - 1.0;
-        ER[3][2] = u;
-        ER[3][3] = H + v * c;
+        if (w == 0) {
+            r[0] = 0.5 * (b2 + wc);
+            r[1] = -0.5 * (b1 * u + zc);
+            r[2] = -0.5 * (b1 * v + ic);
+            r[3] = 0.5 * b1;
+        } else if (w == 1) {
+            r[0] = -e.q2 + e.H;
+            r[1] = u;
+            r[2] = v;
+            r[3] = -1.0;
+        } else if (w == 2) {
+            r[0] = -u;
+            r[1] = 1.0;
+            r[2] = -0.0;
+            r[3] = 0.0;
+        } else {
+            r[0] = 0.5 * (b2 - wc);
+            r[1] = 0.5 * (-b1 * u + zc);
+            r[2] = 0.5 * (-b1 * v + ic);
+            r[3] = 0.5 * b1;
+        }
+    }
+}
+
+// Row `n` of ER (variable n over the four waves); n is a constant after unrolling.
+template <int DIR>
+__device__ __forceinline__ void er_row(const FaceBasis& e, int n, double r[4]) {
+    const double u = e.u, v = e.v, H = e.H, c = e.c, b1 = e.b1;
+    if (n == 0) {
+        r[0] = 1.0;
+        r[1] = b1;
+        r[2] = 0.0;
+        r[3] = 1.0;
+    } else if constexpr (DIR == 1) {
+        if (n == 1) {
+            r[0] = u - 1.0 * c;
+            r[1] = u * b1;
+            r[2] = 0.0;
+            r[3] = u + 1.0 * c;
+        } else if (n == 2) {
+            r[0] = v - 0.0 * c;
+            r[1] = v * b1;
+            r[2] = -1.0;
+            r[3] = v + 0.0 * c;
+        } else {
+            r[0] = H - u * c;
+            r[1] = H * b1 - 1.0;
+            r[2] = -v;
+            r[3] = H + u * c;
+        }
+    } else {
+        if (n == 1) {
+            r[0] = u - 0.0 * c;
+            r[1] = u * b1;
+            r[2] = 1.0;
+            r[3] = u + 0.0 * c;
+        } else if (n == 2) {
+            r[0] = v - 1.0 * c;
+            r[1] = v * b1;
+            r[2] = -0.0;
+            r[3] = v + 1.0 * c;
+        } else {
+            r[0] = H - v * c;
+            r[1] = H * b1 - 1.0;
+            r[2] = u;
+            r[3] = H + v * c;
+        }
     }
 }
 
@@ -302,17 +398,18 @@ __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
                                                  const double st[6][4], double UL[4],
                                                  double UR[4]) {
     if constexpr (CHAR) {
-        double EL[4][4], ER[4][4];
-        face_eigen<DIR>(kc, st[2], st[3], EL, ER);
+        const FaceBasis e = face_basis<DIR>(kc, st[2], st[3]);
         double wl[4], wr[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
+            double EL[4];
+            el_row<DIR>(e, k, EL);
             if constexpr (ORDER == 1) {
                 double l = 0.0, r = 0.0;
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
-                    l += st[2][m] * EL[k][m];
-                    r += st[3][m] * EL[k][m];
+                    l += st[2][m] * EL[m];
+                    r += st[3][m] * EL[m];
                 }
                 wl[k] = l;
                 wr[k] = r;
@@ -322,7 +419,7 @@ __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
                 for (int n = 0; n < 6; n++) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int m = 0; m < 4; m++) acc += st[n][m] * EL[k][m];
+                    for (int m = 0; m < 4; m++) acc += st[n][m] * EL[m];
                     w[n] = acc;
                 }
                 recon_scalar<ORDER, true>(w, delta, kc.r6, wl[k], wr[k]);
@@ -330,11 +427,13 @@ __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
+            double ER[4];
+            er_row<DIR>(e, v, ER);
             double l = 0.0, r = 0.0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                l += wl[k] * ER[v][k];
-                r += wr[k] * ER[v][k];
+                l += wl[k] * ER[k];
+                r += wr[k] * ER[k];
             }
             UL[v] = l;
             UR[v] = r;
