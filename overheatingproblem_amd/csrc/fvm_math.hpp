@@ -51,6 +51,23 @@ __device__ __forceinline__ double quot(double a, const Recip& r) {
     return a / r.b;
 }
 
+// IEEE square root of x in [2^-700, 2^700]: the refinement of the hardware rsq estimate
+// (one Goldschmidt step, two Newton corrections of the root) that the compiler's own
+// double sqrt applies to an operand it does not need to rescale (x >= 2^-767) and that is
+// neither zero nor infinite; bit-identical to sqrt(x) there.
+__device__ __forceinline__ double sqrt_mid(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // Quotients in groups.  The per-quotient test of quot() costs each quotient a divergent
 // branch (an exec-mask save, a skip and a restore: about as many scalar instructions as
 // the quotient has vector ones).  A group of quotients (one WENO weight combination, the
@@ -72,7 +89,43 @@ struct Div {
             return quot(a, r);
         }
     }
+    // sqrt(x), on the fast path through sqrt_mid for x in [2^-700, 2^700].
+    __device__ __forceinline__ double root(double x) {
+        if constexpr (FAST) {
+            ok = ok && x >= 0x1p-700 && x <= 0x1p700;
+            return sqrt_mid(x);
+        } else {
+            return sqrt(x);
+        }
+    }
+    // As operator(), except that the fast path also takes a = +-0 and then returns a zero
+    // whose sign can be wrong (-0 / b gives +0 for b > 0).  Only for quotients whose
+    // zeros cannot reach a result with their sign (see reconstruct_face).
+    __device__ __forceinline__ double zq(double a, const Recip& r) {
+        if constexpr (FAST) {
+            ok = ok && r.ok && (in_safe_range(a) || a == 0.0);
+            const double q = a * r.y;
+            const double rem = fma(-r.b, q, a);
+            return fma(rem, r.y, q);
+        } else {
+            return quot(a, r);
+        }
+    }
 };
+
+// The refined reciprocal and the finished quotient of recip() / quot() without their
+// range tests, for groups whose ranges are established as a whole.
+__device__ __forceinline__ double rcp2(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ double qfin(double a, double b, double y) {
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
 
 // True when `ok` holds in every active lane of the wavefront (a uniform value).
 __device__ __forceinline__ bool all_lanes(bool ok) { return __ballot(!ok) == 0ull; }
@@ -134,15 +187,90 @@ __device__ __forceinline__ double weno5_combine(double a1, double a2, double a3,
 
 // WENO5-Z (Borges et al. 2008, exponent 1, eps 1e-15): WENO5Z_L/R,
 // include/Reconstruction.h:1519-1624.  (v1..v5) in upwind order.
-__device__ __forceinline__ double weno5z(double v1, double v2, double v3, double v4, double v5,
-                                         const Recip& r6) {
+//
+// FAST = false is the reference's arithmetic (IEEE quotients).  FAST = true gives the same
+// value whenever it leaves q.ok set, and the caller reruns FAST = false when any lane
+// cleared it.  It takes the caller's word (q.ok on entry) that |v| <= 2^120, from which
+// every range it needs follows:
+//  * 2 v and 4 v are exact, so fma(-2, v2, v1) rounds like v1 - 2.0 * v2 (one operation
+//    instead of two, 8 per value);
+//  * beta is a sum of squares, 0 <= beta <= 34 (2^120)^2 < 2^246, so d = beta + eps lies
+//    in [1e-15, 2^246];
+//  * tau = |beta1 - beta3| <= 2^246: tau / d is exact for tau >= 2^-300, and below that
+//    any value under 2^-54 gives the reference's 1.0 + tau / d == 1.0, as the fast
+//    quotient does (it is at most 2^-249);
+//  * each alpha is at least 0.1 (0.1 (1 + x), x >= 0) and the sum s at most
+//    1 + 2^246 / 1e-15 < 2^296, so the weights alpha / s are exact;
+//  * each weight is at least 0.1 / 2^296 and each candidate polynomial at most 20 2^120
+//    in magnitude, so a sixth t / 6 is exact for t = 0 (up to the sign, Div::zq: WENO5-Z
+//    runs in the characteristic branch only) or |t| >= 2^-300: the one test left.
+template <bool FAST>
+__device__ __forceinline__ double weno5z_t(double v1, double v2, double v3, double v4, double v5,
+                                           const Recip& r6, Div<FAST>& q) {
     const double eps = 1.0e-15;
-    const Beta5 b = beta5(v1, v2, v3, v4, v5);
-    const double tau = fabs(b.b1 - b.b3);
-    const double a1 = 0.1 * (1.0 + tau / (b.b1 + eps));
-    const double a2 = 0.6 * (1.0 + tau / (b.b2 + eps));
-    const double a3 = 0.3 * (1.0 + tau / (b.b3 + eps));
-    return weno5_combine(a1, a2, a3, v1, v2, v3, v4, v5, r6);
+    if constexpr (!FAST) {
+        const Beta5 b = beta5(v1, v2, v3, v4, v5);
+        const double tau = fabs(b.b1 - b.b3);
+        const double a1 = 0.1 * (1.0 + tau / (b.b1 + eps));
+        const double a2 = 0.6 * (1.0 + tau / (b.b2 + eps));
+        const double a3 = 0.3 * (1.0 + tau / (b.b3 + eps));
+        return weno5_combine_t(a1, a2, a3, v1, v2, v3, v4, v5, r6, q);
+    } else {
+        const double t1 = fma(-2.0, v2, v1) + v3;
+        const double s1 = fma(-4.0, v2, v1) + 3.0 * v3;
+        const double t2 = fma(-2.0, v3, v2) + v4;
+        const double t3 = fma(-2.0, v4, v3) + v5;
+        const double s3 = fma(-4.0, v4, 3.0 * v3) + v5;
+        const double b1 = 13.0 / 12.0 * t1 * t1 + 0.25 * s1 * s1;
+        const double b2 = 13.0 / 12.0 * t2 * t2 + 0.25 * (v2 - v4) * (v2 - v4);
+        const double b3 = 13.0 / 12.0 * t3 * t3 + 0.25 * s3 * s3;
+        const double tau = fabs(b1 - b3);
+        const double d1 = b1 + eps, d2 = b2 + eps, d3 = b3 + eps;
+        const double a1 = 0.1 * (1.0 + qfin(tau, d1, rcp2(d1)));
+        const double a2 = 0.6 * (1.0 + qfin(tau, d2, rcp2(d2)));
+        const double a3 = 0.3 * (1.0 + qfin(tau, d3, rcp2(d3)));
+        const double sum = a1 + a2 + a3;
+        const double ys = rcp2(sum);
+        const double p1 = fma(2.0, v1, -(7.0 * v2)) + 11.0 * v3;
+        const double p2 = fma(2.0, v4, -v2 + 5.0 * v3);
+        const double p3 = fma(2.0, v3, 5.0 * v4) - v5;
+        const double x1 = qfin(a1, sum, ys) * p1;
+        const double x2 = qfin(a2, sum, ys) * p2;
+        const double x3 = qfin(a3, sum, ys) * p3;
+        q.ok = q.ok && (fabs(x1) >= 0x1p-300 || x1 == 0.0) && (fabs(x2) >= 0x1p-300 || x2 == 0.0) &&
+               (fabs(x3) >= 0x1p-300 || x3 == 0.0);
+        return qfin(x1, 6.0, r6.y) + qfin(x2, 6.0, r6.y) + qfin(x3, 6.0, r6.y);
+    }
+}
+
+// The two WENO5-Z values of a characteristic field's six stencil values, each under its
+// own vote.
+__device__ __forceinline__ void weno5z_pair(const double s[6], const Recip& r6, double& left,
+                                            double& right) {
+    // |v| <= 2^120 for all six (the sum of magnitudes bounds each; a NaN or an infinity
+    // fails the test)
+    double m = fabs(s[0]);
+#pragma unroll
+    for (int n = 1; n < 6; n++) m += fabs(s[n]);
+    const bool bounded = m <= 0x1p120;
+    {
+        Div<true> f;
+        f.ok = bounded;
+        left = weno5z_t(s[0], s[1], s[2], s[3], s[4], r6, f);
+        if (!all_lanes(f.ok)) {
+            Div<false> c;
+            left = weno5z_t(s[0], s[1], s[2], s[3], s[4], r6, c);
+        }
+    }
+    {
+        Div<true> f;
+        f.ok = bounded;
+        right = weno5z_t(s[5], s[4], s[3], s[2], s[1], r6, f);
+        if (!all_lanes(f.ok)) {
+            Div<false> c;
+            right = weno5z_t(s[5], s[4], s[3], s[2], s[1], r6, c);
+        }
+    }
 }
 
 // WENO5-JS: WENO5_L/R, include/Reconstruction.h:1408-1508.
@@ -203,8 +331,7 @@ __device__ __forceinline__ void recon_scalar(const double s[6], double delta, co
         right = weno3(s[4], s[3], s[2]);
     } else {
         if constexpr (CHAR) {
-            left = weno5z(s[0], s[1], s[2], s[3], s[4], r6);
-            right = weno5z(s[5], s[4], s[3], s[2], s[1], r6);
+            weno5z_pair(s, r6, left, right);
         } else {
             left = weno5js(s[0], s[1], s[2], s[3], s[4], r6);
             right = weno5js(s[5], s[4], s[3], s[2], s[1], r6);
@@ -232,8 +359,8 @@ __device__ __forceinline__ void primitive_h(const Consts& k, const double c[4], 
                                             double& H, Recip& rr, Div<FAST>& q) {
     rr = recip(c[0]);
     const double rho = c[0];
-    const double u = q(c[1], rr);
-    const double v = q(c[2], rr);
+    const double u = q.zq(c[1], rr);
+    const double v = q.zq(c[2], rr);
     p[0] = rho;
     p[1] = u;
     p[2] = v;
@@ -250,8 +377,12 @@ __device__ __forceinline__ void primitive_h(const Consts& k, const double c[4], 
 // (FaceBasis) and forms each row where it is used: a row of EL per projected wave, a row
 // of ER per back-projected variable.  Every entry is the same expression as in the
 // reference and is formed once per face, as when the matrices are held; holding nine
-// scalars instead of 32 entries is what keeps the characteristic kernels below 128
-// VGPRs.
+// scalars instead of 32 entries removed the fused y sweep's register spill.
+//
+// The velocity quotients (cell and Roe-averaged u, v and u/c or v/c) take Div::zq: a
+// velocity component enters the entries only through sums with nonzero terms, squares
+// and products, so the sign of a zero velocity can only change the sign of a zero entry,
+// which reconstruct_face's sums cannot pass on.
 struct FaceBasis {
     double u, v, H, q2, c, b1, b2, ic, wc;  // wc = u/c (DIR 1) or v/c (DIR 2)
 };
@@ -263,21 +394,21 @@ __device__ __forceinline__ FaceBasis face_basis_t(const Consts& k, const double 
     Recip ra, rb;
     primitive_h(k, ca, a, Ha, ra, quot);
     primitive_h(k, cb, b, Hb, rb, quot);
-    const double D = sqrt(quot(b[0], ra));
+    const double D = quot.root(quot(b[0], ra));
     const double D1 = D + 1.0;
     const Recip rd = recip(D1);
     FaceBasis e;
-    e.u = quot(a[1] + D * b[1], rd);
-    e.v = quot(a[2] + D * b[2], rd);
+    e.u = quot.zq(a[1] + D * b[1], rd);
+    e.v = quot.zq(a[2] + D * b[2], rd);
     e.H = quot(Ha + D * Hb, rd);
     e.q2 = e.u * e.u + e.v * e.v;
     const double c2 = k.gm1 * (e.H - 0.5 * e.q2);
-    e.c = sqrt(c2);
+    e.c = quot.root(c2);
     e.b1 = k.gm1 / c2;
     e.b2 = 1.0 + e.b1 * e.q2 - e.b1 * e.H;
     const Recip rc = recip(e.c);
     e.ic = quot(1.0, rc);
-    e.wc = quot(DIR == 1 ? e.u : e.v, rc);
+    e.wc = quot.zq(DIR == 1 ? e.u : e.v, rc);
     return e;
 }
 
@@ -393,6 +524,16 @@ __device__ __forceinline__ void er_row(const FaceBasis& e, int n, double r[4]) {
 // project w = L.U per wave (sum from 0.0 in var order), reconstruct, back-project
 // U = R.w (sum from 0.0 in wave order): include/Reconstruction.h:909-967 and its
 // Godunov/TVD/WENO3 twins; component branch: :1031-1073 and twins.
+//
+// Signs of zeros.  The back-projection's sums start from +0.0, and +0.0 + (-0) == +0.0, so
+// they give the same result for terms that differ only in the signs of zeros.  Every
+// reconstruction here maps values that differ only in the signs of zeros to values that
+// differ only in the signs of zeros (sums, products, fabs, comparisons, and quotients by
+// denominators that are never zero: beta + eps, the weight sum, 6, delta, (eps + s)^2,
+// |a| + |b| + 1e-6).  Neither the zero signs of the entries of EL and ER nor those of
+// the projected values w can therefore change UL or UR, which lets the projection start
+// from its first product instead of 0.0 + that product, and the WENO5-Z sixths and the
+// basis's velocity quotients take Div::zq.
 template <int DIR, int ORDER, bool CHAR>
 __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
                                                  const double st[6][4], double UL[4],
@@ -405,9 +546,9 @@ __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
             double EL[4];
             el_row<DIR>(e, k, EL);
             if constexpr (ORDER == 1) {
-                double l = 0.0, r = 0.0;
+                double l = st[2][0] * EL[0], r = st[3][0] * EL[0];
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
+                for (int m = 1; m < 4; m++) {
                     l += st[2][m] * EL[m];
                     r += st[3][m] * EL[m];
                 }
@@ -417,9 +558,9 @@ __device__ __forceinline__ void reconstruct_face(const Consts& kc, double delta,
                 double w[6];
 #pragma unroll
                 for (int n = 0; n < 6; n++) {
-                    double acc = 0.0;
+                    double acc = st[n][0] * EL[0];
 #pragma unroll
-                    for (int m = 0; m < 4; m++) acc += st[n][m] * EL[m];
+                    for (int m = 1; m < 4; m++) acc += st[n][m] * EL[m];
                     w[n] = acc;
                 }
                 recon_scalar<ORDER, true>(w, delta, kc.r6, wl[k], wr[k]);
@@ -467,6 +608,22 @@ __device__ __forceinline__ Side make_side(const Consts& k, const double c[4]) {
     return s;
 }
 
+// make_side with 1 / rho from rcp2 (exact for rho in [2^-300, 2^300]).
+__device__ __forceinline__ Side make_side_fast(const Consts& k, const double c[4]) {
+    Side s;
+    s.r = c[0];
+    s.ru = c[1];
+    s.rv = c[2];
+    s.E = c[3];
+    s.ir = qfin(1.0, s.r, rcp2(s.r));
+    s.u = s.ru * s.ir;
+    s.v = s.rv * s.ir;
+    s.u2 = s.u * s.u;
+    s.v2 = s.v * s.v;
+    s.p = (s.E - 0.5 * s.r * (s.u2 + s.v2)) * k.gm1;
+    return s;
+}
+
 template <int DIR>
 __device__ __forceinline__ void phys_flux(const Side& s, double H, double F[4]) {
     if constexpr (DIR == 1) {
@@ -484,32 +641,54 @@ __device__ __forceinline__ void phys_flux(const Side& s, double H, double F[4]) 
 
 // HLLC with Einfeldt speeds, include/scheme.h:690-1001 (per-direction round-off:
 // sound speed :745-746 vs :895-896, Einfeldt eta :777 vs :929).
-template <int DIR>
-__device__ __forceinline__ int hllc(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+//
+// FAST: the reciprocals of rho and of sqrt(rho_L) + sqrt(rho_R), the Einfeldt quotient and
+// the five square roots come from rcp2 / sqrt_mid, which give the IEEE results when both
+// densities lie in [2^-290, 2^290] (then every reciprocal and the quotient have their
+// operands in [2^-300, 2^300]) and the sound speeds' and the Einfeldt speed's squares in
+// [2^-700, 2^700]; `ok` says whether they did, and hllc() reruns FAST = false otherwise.
+template <int DIR, bool FAST>
+__device__ __forceinline__ int hllc_t(const Consts& k, const double cl[4], const double cr[4],
+                                      double F[4], bool& ok) {
     const double g = k.g, gm1 = k.gm1, gr = k.gr;
-    const Side L = make_side(k, cl), R = make_side(k, cr);
+    auto root = [&](double x) {
+        if constexpr (FAST) {
+            ok = ok && x >= 0x1p-700 && x <= 0x1p700;
+            return sqrt_mid(x);
+        } else {
+            return sqrt(x);
+        }
+    };
+    if constexpr (FAST)
+        ok = cl[0] >= 0x1p-290 && cl[0] <= 0x1p290 && cr[0] >= 0x1p-290 && cr[0] <= 0x1p290;
+    const Side L = FAST ? make_side_fast(k, cl) : make_side(k, cl);
+    const Side R = FAST ? make_side_fast(k, cr) : make_side(k, cr);
     const double HL = 0.5 * (L.u2 + L.v2) + gr * (L.p * L.ir);
     const double HR = 0.5 * (R.u2 + R.v2) + gr * (R.p * R.ir);
     double aL, aR;
     if constexpr (DIR == 1) {
-        aL = sqrt((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
-        aR = sqrt((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
+        aL = root((g - 1.0) * (HL - 0.5 * (L.u2 + L.v2)));
+        aR = root((g - 1.0) * (HR - 0.5 * (R.u2 + R.v2)));
     } else {
-        aL = sqrt(g * L.p * L.ir);
-        aR = sqrt(g * R.p * R.ir);
+        aL = root(g * L.p * L.ir);
+        aR = root(g * R.p * R.ir);
     }
     double FL[4], FR[4];
     phys_flux<DIR>(L, HL, FL);
     phys_flux<DIR>(R, HR, FR);
-    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    // rho in [2^-290, 2^290]: sL, sR in [2^-145, 2^145], ss in [2^-145, 2^146]
+    const double sL = FAST ? sqrt_mid(L.r) : sqrt(L.r), sR = FAST ? sqrt_mid(R.r) : sqrt(R.r);
     const double ss = sL + sR;
-    const double iss = 1.0 / ss;
+    const double iss = FAST ? qfin(1.0, ss, rcp2(ss)) : 1.0 / ss;
     double eta;
     if constexpr (DIR == 1) eta = 0.5 * sL * sR * iss * iss;
-    else eta = 0.5 * sL * sR / (ss * ss);
+    else {
+        const double s2 = ss * ss;  // in [2^-290, 2^292]
+        eta = FAST ? qfin(0.5 * sL * sR, s2, rcp2(s2)) : 0.5 * sL * sR / s2;
+    }
     const double qL = DIR == 1 ? L.u : L.v, qR = DIR == 1 ? R.u : R.v;
     const double nbar = ((DIR == 1 ? sL * L.u + sR * R.u : sL * L.v + sR * R.v)) * iss;
-    const double d = sqrt((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
+    const double d = root((sL * aL * aL + sR * aR * aR) * iss + eta * (qR - qL) * (qR - qL));
     const double sl = nbar - d, sr = nbar + d;
     if (sl >= 0.0) {
 #pragma unroll
@@ -549,6 +728,14 @@ __device__ __forceinline__ int hllc(const Consts& k, const double cl[4], const d
         F[3] = FR[3] + sr * (m * e - R.E);
     }
     return 0;
+}
+
+template <int DIR>
+__device__ __forceinline__ int hllc(const Consts& k, const double cl[4], const double cr[4], double F[4]) {
+    bool ok = true;
+    const int r = hllc_t<DIR, true>(k, cl, cr, F, ok);
+    if (all_lanes(ok)) return r;
+    return hllc_t<DIR, false>(k, cl, cr, F, ok);
 }
 
 // HLL with Einfeldt speeds, include/scheme.h:400-674.  The reference prints and
