@@ -109,8 +109,23 @@ def traffic_markdown(t: dict, name: str, commands: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def agent(pmc_dir: str) -> dict:
+    """CU, SIMD and XCD counts of the GPU agent of a rocprofv3 run."""
+    for r in csv.DictReader(open(_one(pmc_dir, "agent_info.csv"))):
+        if r.get("Name", "").startswith("gfx"):
+            return {"cus": int(r["Cu_Count"]), "simds": int(r["Simd_Count"]),
+                    "xcds": int(r["Num_Xcc"]), "name": r["Name"]}
+    raise ValueError(f"no GPU agent in {pmc_dir}")
+
+
 def valu(pmc_dir: str) -> dict:
-    """Per-dispatch SQ instruction counts (SQ_INSTS_VALU etc.) of each kernel."""
+    """Per-dispatch SQ instruction counts (SQ_INSTS_VALU etc.) of each kernel, and, when
+    the run also counted SQ_ACTIVE_INST_VALU, SQ_WAVE_CYCLES and GRBM_GUI_ACTIVE:
+      valu_busy       = SQ_ACTIVE_INST_VALU / (CUs x cycles): the fraction of SIMD cycles
+                        issuing VALU work (the counter is in quad-cycles, 4 SIMDs per CU);
+      waves_per_simd  = 4 SQ_WAVE_CYCLES / (SIMDs x cycles): resident wavefronts;
+    with cycles = GRBM_GUI_ACTIVE / XCDs (the counter is summed over the XCDs)."""
+    ag = agent(pmc_dir)
     out = {}
     for k, cs in counter_means(pmc_dir).items():
         if k.startswith("__amd") or "SQ_INSTS_VALU" not in cs:
@@ -118,19 +133,32 @@ def valu(pmc_dir: str) -> dict:
         rec = {c.lower() + "_per_dispatch": v for c, (v, _) in cs.items()}
         rec["valu_wave_instr_per_dispatch"] = cs["SQ_INSTS_VALU"][0]
         rec["dispatches"] = cs["SQ_INSTS_VALU"][1]
+        if all(c in cs for c in ("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "GRBM_GUI_ACTIVE")):
+            cycles = cs["GRBM_GUI_ACTIVE"][0] / ag["xcds"]
+            if cycles > 0:
+                rec["valu_busy"] = cs["SQ_ACTIVE_INST_VALU"][0] / (ag["cus"] * cycles)
+                rec["waves_per_simd"] = 4.0 * cs["SQ_WAVE_CYCLES"][0] / (ag["simds"] * cycles)
         out[k] = rec
     return out
 
 
 def valu_markdown(v: dict, name: str, command: str) -> str:
     cols = sorted({c for rec in v.values() for c in rec if c.endswith("_per_dispatch")})
+    derived = [c for c in ("valu_busy", "waves_per_simd") if any(c in r for r in v.values())]
     lines = [f"# SQ instruction counters per dispatch: {name}", "",
-             "Command (from the repository root):", "", "```", command, "```", "",
-             "| kernel | dispatches | " + " | ".join(c.replace("_per_dispatch", "") for c in cols)
-             + " |", "|---|---:|" + "---:|" * len(cols)]
+             "Command (from the repository root):", "", "```", command, "```", ""]
+    if derived:
+        lines += ["valu_busy = SQ_ACTIVE_INST_VALU / (CUs x cycles) and waves_per_simd = "
+                  "4 SQ_WAVE_CYCLES / (SIMDs x cycles), with cycles = GRBM_GUI_ACTIVE / XCDs "
+                  "(SQ_ACTIVE_INST_VALU and SQ_WAVE_CYCLES count quad-cycles; GRBM_GUI_ACTIVE "
+                  "is summed over the 8 XCDs).", ""]
+    lines += ["| kernel | dispatches | " + " | ".join(derived + [c.replace("_per_dispatch", "")
+                                                          for c in cols]) + " |",
+              "|---|---:|" + "---:|" * (len(derived) + len(cols))]
     for k, rec in v.items():
         lines.append(f"| {k} | {rec['dispatches']} | "
-                     + " | ".join(f"{rec.get(c, 0):.4g}" for c in cols) + " |")
+                     + " | ".join([f"{rec[c]:.3f}" if c in rec else "" for c in derived]
+                                  + [f"{rec.get(c, 0):.4g}" for c in cols]) + " |")
     return "\n".join(lines) + "\n"
 
 
