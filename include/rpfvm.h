@@ -82,11 +82,19 @@ enum {
 /* rpfvm_config.debug_flags bits. */
 enum {
     RPFVM_DEBUG_MIXED_ALL_FP64 = 1,  /* every mixed-form face through the FP64 recompute list */
-    RPFVM_DEBUG_LARGE_ARRAY_FORMS = 2 /* the kernel forms used when an array (4 planes)
+    RPFVM_DEBUG_LARGE_ARRAY_FORMS = 2, /* the kernel forms used when an array (4 planes)
                                         reaches 4 GB, at any size: the fused y sweep one row
                                         per wavefront instead of the LDS-staged marching form,
                                         global instead of buffer addressing in the mixed x
-                                        sweep; identical results */
+                                        sweep, the mixed form's two-sweep stage; identical
+                                        results */
+    RPFVM_DEBUG_TWO_SWEEP = 4,       /* the mixed form's stage as an x sweep followed by the
+                                        fused y sweep (the default for characteristic
+                                        WENO5); identical results */
+    RPFVM_DEBUG_ONE_PASS = 8         /* the mixed form's stage as one pass (x faces, y faces
+                                        and update in one kernel; the default for the other
+                                        reconstructions) also for characteristic WENO5;
+                                        identical results */
 };
 
 /* Error codes (< 0). */
@@ -238,7 +246,10 @@ int rpfvm_stream(rpfvm_ctx *ctx, void **stream);
 enum {
     RPFVM_KCLASS_BC = 0,       /* ghost fill */
     RPFVM_KCLASS_FLUX_X = 1,   /* x-face reconstruction + Riemann flux */
-    RPFVM_KCLASS_FLUX_Y = 2,   /* y-face reconstruction + Riemann flux */
+    RPFVM_KCLASS_FLUX_Y = 2,   /* y-face reconstruction + Riemann flux (fused with the
+                                  residual + stage combine where the stage writes a
+                                  separate output; the mixed form's one-pass stage, x faces
+                                  included, is timed here too) */
     RPFVM_KCLASS_UPDATE = 3,   /* residual + RK stage combine */
     RPFVM_KCLASS_DT = 4,       /* CFL reduction */
     RPFVM_KCLASS_COUNT = 5

@@ -25,6 +25,8 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 FAULT_HLL_BAD_STATE, FAULT_NONFINITE_DT, FAULT_MIXED_FALLBACK, FAULT_POSITIVITY = 1, 2, 4, 8
 DEBUG_MIXED_ALL_FP64 = 1
 DEBUG_LARGE_ARRAY_FORMS = 2
+DEBUG_TWO_SWEEP = 4
+DEBUG_ONE_PASS = 8
 
 # The reference's default inflow / fixed-value states: a Mach-3 stream of
 # rho = 1.4, u = 3, p = 1 in conserved form (include/Boundary_Condition.h:49,

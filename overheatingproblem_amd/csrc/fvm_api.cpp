@@ -226,7 +226,11 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     const size_t ncell = (size_t)c.nx * c.ny;
     const size_t cell_bytes = ((ncell * 4 + 255) / 256) * 256;
     const size_t rep_bytes = c.positivity ? 5 * cell_bytes + 256 : 0;
-    const size_t face_bytes = (((size_t)(c.nx + 1) * (c.ny + 1) * 4 + 255) / 256) * 256;
+    // FP64 recompute lists: every face, plus the faces two wavefronts both compute (the
+    // first x-face row of each block of >= 4 rows in the one-pass stage, the strip-edge y
+    // faces of the fused y sweeps)
+    const size_t fix_cap = (size_t)(c.nx + 2 + (c.nx + 3) / 4) * (size_t)(c.ny + 2 + (c.ny + kStrip - 1) / kStrip);
+    const size_t face_bytes = ((fix_cap * 4 + 255) / 256) * 256;
     const size_t fix_bytes = c.precision == RPFVM_PRECISION_MIXED ? 2 * face_bytes + 256 : 0;
     const size_t slot_bytes = (((size_t)dt_slot_count(x->g) * 2 * sizeof(unsigned long long) + 255) / 256) * 256;
     const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes + slot_bytes;
@@ -257,6 +261,7 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
         x->fix.cnt = reinterpret_cast<unsigned*>(r);
         x->fix.list[0] = reinterpret_cast<int*>(r + 256);
         x->fix.list[1] = reinterpret_cast<int*>(r + 256 + face_bytes);
+        x->fix.cap = (unsigned)fix_cap;
     }
     x->slots.slot = reinterpret_cast<unsigned long long*>(q + small + rep_bytes + fix_bytes);
     x->slots.nstrips = (c.ny + kStrip - 1) / kStrip;
@@ -348,21 +353,49 @@ static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
         launch_dt(x->U, x->g, x->ph, x->partial, x->scal, clock, x->faults, x->stream);
 }
 
-// One stage: ghost fill of Y, x-face fluxes, then the y faces fused with the residual
-// and stage combine into out (separately when out aliases Y, which the fused kernel's
-// neighbour reads forbid), then the positivity safeguard.
+// Whether a stage writing a separate output runs as one pass: the form has it (mixed),
+// the arrays take buffer offsets, and the schedule is the faster one for the
+// reconstruction.  Measured at Noh 4096^2 (ms per step, one-pass / two-sweep): Godunov
+// 1.63 / 2.07, TVD 2.27 / 2.37, WENO3 2.16 / 2.42, characteristic WENO5 2.88 / 2.84 (its
+// two face computations need a 2-wave register budget in one kernel).  The debug flags
+// force either schedule.
+static bool one_pass(const rpfvm_ctx* x) {
+    if (!x->flux.stage || !use_buffer_forms(x->g, x->ph)) return false;
+    const int f = x->cfg.debug_flags;
+    if (f & RPFVM_DEBUG_TWO_SWEEP) return false;
+    if (f & RPFVM_DEBUG_ONE_PASS) return true;
+    return !(x->cfg.recon_order == 5 && x->cfg.characteristic);
+}
+
+// One stage: ghost fill of Y, then either the one-pass stage (x faces, y faces, residual
+// and stage combine in one kernel) or the x-face fluxes followed by the y faces fused with
+// the residual and stage combine into out (separately when out aliases Y, which the fused
+// kernel's neighbour reads forbid), then the positivity safeguard.
 // `last`: the step's last stage, which leaves the next step's wave-speed slots (fused
 // path only; returns whether it did).
 static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
                           int kind, double t_stage, bool last) {
     const unsigned stamp = ++x->stamp;
     enqueue_bc(x, Y, t_stage);
+    const bool fused = out != Y;
+    const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
+    if (fused && one_pass(x)) {
+        {
+            Timed t(x, RPFVM_KCLASS_FLUX_Y);
+            x->flux.stage(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds,
+                          x->faults, x->stream);
+        }
+        if (x->rp.cnt) {
+            Timed t(x, RPFVM_KCLASS_UPDATE);
+            x->flux.repair1(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds,
+                            x->faults, x->stream);
+        }
+        return ds.slot != nullptr;
+    }
     {
         Timed t(x, RPFVM_KCLASS_FLUX_X);
         x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
     }
-    const bool fused = out != Y;
-    const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
     if (fused) {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
         x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds,

@@ -417,7 +417,7 @@ static void launch_yflux_update(const double* Un, const double* Y, double* out, 
 template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair pick3() {
     return FluxPair{&launch_flux<1, ORDER, CHAR, SOLVER>, &launch_flux<2, ORDER, CHAR, SOLVER>,
-                    &launch_yflux_update<ORDER, CHAR, SOLVER>, &launch_repair};
+                    &launch_yflux_update<ORDER, CHAR, SOLVER>, &launch_repair, nullptr, nullptr};
 }
 
 template <int ORDER, bool CHAR>

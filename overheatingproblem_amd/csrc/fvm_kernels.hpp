@@ -85,10 +85,13 @@ struct Repair {
 };
 constexpr int kRepairRounds = 8;
 
-// Faces of the mixed form that need the FP64 path (x and y lists, reset alternately).
+// Faces of the mixed form that need the FP64 path.  The two-sweep stage resets the lists
+// alternately (each direction's fix-up clears the other's count); the one-pass stage
+// fills both and its cell fix-up clears both (cnt[2] counts its finished workgroups).
 struct FaceFix {
-    unsigned* cnt;  // [0] x-face count, [1] y-face count
+    unsigned* cnt;  // [0] x-face count, [1] y-face count, [2] workgroups done
     int* list[2];   // face offsets i*ld + j
+    unsigned cap;   // entries per list (every face, plus the faces two wavefronts share)
 };
 
 // Wave-speed maxima of the last stage's new state, one slot per wavefront row of the fused
@@ -118,10 +121,19 @@ using RepairLaunch = void (*)(const double* Un, const double* Y, double* out, do
                               const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
                               unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
 
+// The whole stage in one pass (fvm_onepass.hpp): x faces, y faces, residual and stage
+// combine, then the fix-ups of deferred faces (out must not alias Y; buffer forms only).
+using StageLaunch = void (*)(const double* Un, const double* Y, double* out, double* F, double* G,
+                             const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                             const FaceFix& fix, unsigned stamp, const DtSlots& ds,
+                             unsigned* faults, hipStream_t s);
+
 struct FluxPair {
     FluxLaunch x, y;
     FusedLaunch yu;
     RepairLaunch repair;
+    StageLaunch stage;      // null: the form has no one-pass stage
+    RepairLaunch repair1;   // the safeguard's rounds after a one-pass stage
 };
 
 FluxPair select_flux(int order, bool characteristic, int solver);

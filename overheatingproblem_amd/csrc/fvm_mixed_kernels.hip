@@ -10,12 +10,21 @@
 
 #include "fvm_buffer.hpp"
 #include "fvm_fused.hpp"
+#include "fvm_onepass.hpp"
 #include "fvm_repair.hpp"
 #include "fvm_kernels.hpp"
 #include "fvm_math.hpp"
 #include "fvm_mixed.hpp"
 
 namespace fvm {
+
+// Appends face c0 to the FP64 recompute list of direction d (0: x, 1: y).  The lists hold
+// every face of the grid plus those two wavefronts both compute (FaceFix::cap), so the
+// capacity test never fails; it keeps a stray entry from writing past the list.
+__device__ __forceinline__ void defer_face(const FaceFix& fix, int d, size_t c0) {
+    const unsigned k = atomicAdd(&fix.cnt[d], 1u);
+    if (k < fix.cap) fix.list[d][k] = (int)c0;
+}
 
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
 __device__ __forceinline__ int face_fp64(const Phys& ph, const double st[6][4], double f[4]) {
@@ -63,8 +72,7 @@ __global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
     double UL[4], UR[4], f[4];
     mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
     if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR))) {
-        const unsigned k = atomicAdd(&fix.cnt[DIR - 1], 1u);
-        fix.list[DIR - 1][k] = (int)c0;
+        defer_face(fix, DIR - 1, c0);
         return;
     }
     mx::riemann<DIR, SOLVER>(ph.gamma, UL, UR, f);
@@ -78,7 +86,7 @@ template <int DIR, int ORDER, bool CHAR, int SOLVER>
 __global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
                                                   double* __restrict__ F, Geom g, Phys ph,
                                                   FaceFix fix, unsigned* __restrict__ faults) {
-    const unsigned n = fix.cnt[DIR - 1];
+    const unsigned n = min(fix.cnt[DIR - 1], fix.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         fix.cnt[2 - DIR] = 0;
         if (n) atomicOr(faults, 4u);
@@ -115,10 +123,23 @@ struct FaceOpMx {
         mx::riemann<2, SOLVER>(ph.gamma, UL, UR, f);
         return true;
     }
-    __device__ __forceinline__ void defer(size_t c0) {
-        const unsigned k = atomicAdd(&fix.cnt[1], 1u);
-        fix.list[1][k] = (int)c0;
+    __device__ __forceinline__ void defer(size_t c0) { defer_face(fix, 1, c0); }
+};
+
+// The x faces of the one-pass stage (k_stage_mixed): k_flux_mixed's face arithmetic.
+template <int ORDER, bool CHAR, int SOLVER>
+struct FaceOpMxX {
+    const Phys& ph;
+    FaceFix fix;
+    __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
+        double UL[4], UR[4];
+        mx::reconstruct_face<1, ORDER, CHAR>(ph.gamma, ph.dx, st, UL, UR);
+        if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
+            return false;
+        mx::riemann<1, SOLVER>(ph.gamma, UL, UR, f);
+        return true;
     }
+    __device__ __forceinline__ void defer(size_t c0) { defer_face(fix, 0, c0); }
 };
 
 // SLOTS: the last stage, which also leaves the next step's wave-speed maxima in `ds`.
@@ -162,7 +183,7 @@ __global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const doub
                                                     const double* __restrict__ G, Geom g, Phys ph,
                                                     double dt, int kind, Repair rp, FaceFix fix,
                                                     unsigned tstamp, DtSlots ds) {
-    const unsigned n = fix.cnt[1];
+    const unsigned n = min(fix.cnt[1], fix.cap);
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n; q += gridDim.x * blockDim.x) {
         const size_t c0 = (size_t)fix.list[1][q >> 1];
         const int i = (int)(c0 / g.ld);
@@ -269,12 +290,155 @@ static void launch_repair_mx(const double* Un, const double* Y, double* out, dou
                                                    faults, s);
 }
 
+// ------------------------------------------------------------- one-pass stage
+// x faces, y faces, residual and stage combine in one marching kernel (fvm_onepass.hpp):
+// the x-face fluxes never reach HBM.  Faces whose FP32 states are not admissible go to the
+// two FP64 lists; k_flux_fix_xy recomputes them and k_fix_cells_xy updates their cells
+// again.  The 13 KB LDS ring of each wavefront allows 12 per CU (3 per SIMD);
+// op_occupancy sets the register budget.
+template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
+__global__ __launch_bounds__(64 * kOpWaves, (op_occupancy<ORDER, CHAR>())) void k_stage_mixed(const double* Un, const double* Y,
+                                                                  double* out, double* F, double* G,
+                                                                  Geom g, Phys ph, double dt, int kind,
+                                                                  Repair rp, FaceFix fix, DtSlots ds) {
+    __shared__ double lds[kOpWaves * kOpLds];
+    FaceOpMxX<ORDER, CHAR, SOLVER> opx{ph, fix};
+    FaceOpMx<ORDER, CHAR, SOLVER> opy{ph, fix};
+    stage_march(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, opx, opy, lds);
+}
+
+// FP64 recomputation of both lists' faces (x faces into F, y faces into G).
+template <int ORDER, bool CHAR, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux_fix_xy(const double* __restrict__ Y, double* F,
+                                                     double* G, Geom g, Phys ph, FaceFix fix,
+                                                     unsigned* __restrict__ faults) {
+    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n0 | n1)) atomicOr(faults, 4u);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n0 + n1; q += gridDim.x * blockDim.x) {
+        const bool xf = q < n0;
+        const size_t c0 = (size_t)(xf ? fix.list[0][q] : fix.list[1][q - n0]);
+        const size_t step = xf ? (size_t)g.ld : 1;
+        double st[6][4], f[4];
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (m - 2) * step];
+        const int bad = xf ? face_fp64<1, ORDER, CHAR, SOLVER>(ph, st, f)
+                           : face_fp64<2, ORDER, CHAR, SOLVER>(ph, st, f);
+        if (bad) atomicOr(faults, 1u);
+        double* D = xf ? F : G;
+#pragma unroll
+        for (int v = 0; v < 4; v++) D[(size_t)v * g.plane + c0] = f[v];
+    }
+}
+
+// The two cells of every listed face, updated again from F and G (cell_update); then the
+// last workgroup to finish clears both lists for the next stage.
+__global__ __launch_bounds__(256) void k_fix_cells_xy(const double* Un, const double* Y, double* out,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ G, Geom g, Phys ph,
+                                                      double dt, int kind, Repair rp, FaceFix fix,
+                                                      unsigned tstamp, DtSlots ds) {
+    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (n0 + n1); q += gridDim.x * blockDim.x) {
+        int i, j;
+        if (q < 2 * n0) {  // x face (i, j): cells (i, j) and (i + 1, j)
+            const size_t c0 = (size_t)fix.list[0][q >> 1];
+            i = (int)(c0 / g.ld) + (int)(q & 1);
+            j = (int)(c0 % g.ld);
+            if (i < GC || i > g.R - GC - 1) continue;
+        } else {  // y face (i, j): cells (i, j) and (i, j + 1)
+            const unsigned r = q - 2 * n0;
+            const size_t c0 = (size_t)fix.list[1][r >> 1];
+            i = (int)(c0 / g.ld);
+            j = (int)(c0 % g.ld) + (int)(r & 1);
+            if (j < GC || j > g.C - GC - 1) continue;
+        }
+        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
+            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
+            ds.slot[2 * w] = kSlotStale;
+        }
+        if (rp.cnt && !ok) {
+            const int cell = (i - GC) * rp.ny + (j - GC);
+            if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
+                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+                rp.list[0][k] = cell;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&fix.cnt[2], 1u) == gridDim.x - 1) {  // every workgroup has read the counts
+            fix.cnt[0] = 0;
+            fix.cnt[1] = 0;
+            fix.cnt[2] = 0;
+            __threadfence();
+        }
+    }
+}
+
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_stage_mixed(const double* Un, const double* Y, double* out, double* F,
+                               double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                               const Repair& rp, const FaceFix& fix, unsigned stamp,
+                               const DtSlots& ds, unsigned* faults, hipStream_t s) {
+    if (ds.slot)
+        hipLaunchKernelGGL((k_stage_mixed<ORDER, CHAR, SOLVER, true>), stage_march_grid(g),
+                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+    else
+        hipLaunchKernelGGL((k_stage_mixed<ORDER, CHAR, SOLVER, false>), stage_march_grid(g),
+                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+    hipLaunchKernelGGL((k_flux_fix_xy<ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, G, g, ph,
+                       fix, faults);
+    hipLaunchKernelGGL(k_fix_cells_xy, dim3(64), dim3(256), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp,
+                       fix, stamp * 16u + 15u, ds);
+}
+
+// The x-face fluxes of the safeguard's re-updated cells after a one-pass stage, which stores
+// only those its own fix-up reads: face (i, j) between cells (i, j) and (i + 1, j) comes
+// from F when the safeguard replaced it or the FP32 path deferred it, and is otherwise
+// recomputed with the stage's own arithmetic (the value the stage had).
+template <int ORDER, bool CHAR, int SOLVER>
+struct XFacesMx {
+    static constexpr bool kStored = false;
+    __device__ static void face(const double* Y, const double* F, const Geom& g, const Phys& ph,
+                                const Repair& rp, unsigned stamp, int i, int j, double out[4]) {
+        const int col = j - GC;
+        const bool replaced = (i >= GC && rp.rep[(i - GC) * rp.ny + col] == stamp) ||
+                              (i + 1 <= g.R - GC - 1 && rp.rep[(i + 1 - GC) * rp.ny + col] == stamp);
+        const size_t c0 = (size_t)i * g.ld + j;
+        if (!replaced) {
+            double st[6][4];
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (size_t)(n - 2) * g.ld];
+            FaceOpMxX<ORDER, CHAR, SOLVER> op{ph, FaceFix{}};
+            if (op.compute(st, out)) return;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[v] = F[(size_t)v * g.plane + c0];
+    }
+};
+
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_repair1_mx(const double* Un, const double* Y, double* out, double* F, double* G,
+                              const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                              unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
+    launch_repair_t<YFacesMx<ORDER, CHAR, SOLVER>, XFacesMx<ORDER, CHAR, SOLVER>>(
+        Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults, s);
+}
+
 template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair mpick3() {
     return FluxPair{&launch_flux_mixed<1, ORDER, CHAR, SOLVER>,
                     &launch_flux_mixed<2, ORDER, CHAR, SOLVER>,
                     &launch_yflux_update_mixed<ORDER, CHAR, SOLVER>,
-                    &launch_repair_mx<ORDER, CHAR, SOLVER>};
+                    &launch_repair_mx<ORDER, CHAR, SOLVER>,
+                    &launch_stage_mixed<ORDER, CHAR, SOLVER>,
+                    &launch_repair1_mx<ORDER, CHAR, SOLVER>};
 }
 
 template <int ORDER, bool CHAR>

@@ -1,8 +1,9 @@
 // fvm_repair.hpp -- the positivity safeguard's rounds: device phases, kernels and their
-// launcher, templated on where a re-updated cell's y-face fluxes come from (YF):
-// YFacesStored reads them from G (every face stored, as the FP64 form does); the mixed
-// form, whose fused sweep stores only the faces its own fix-up reads, recomputes them
-// (fvm_mixed_kernels.hip).  x-face fluxes are always read from F.
+// launcher, templated on where a re-updated cell's face fluxes come from (YF for y faces,
+// XF for x faces): YFacesStored / XFacesStored read them from G / F (every face stored, as
+// the FP64 form does); the mixed form, whose sweeps store only the faces its own fix-up
+// reads, recomputes the others (fvm_mixed_kernels.hip): the y faces after its fused y
+// sweep, and the x faces too after its one-pass stage.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,6 +16,10 @@ namespace fvm {
 
 // Every y face is in G.
 struct YFacesStored {
+    static constexpr bool kStored = true;
+};
+// Every x face is in F.
+struct XFacesStored {
     static constexpr bool kStored = true;
 };
 
@@ -81,7 +86,7 @@ __device__ __forceinline__ void repair_touch_phase(const Repair& rp, int round, 
     }
 }
 
-template <typename YF>
+template <typename YF, typename XF>
 __device__ __forceinline__ void repair_cells_phase(const double* Un, const double* Y, double* out,
                                                    const double* __restrict__ F,
                                                    const double* __restrict__ G, const Geom& g,
@@ -95,8 +100,22 @@ __device__ __forceinline__ void repair_cells_phase(const double* Un, const doubl
         const int c = rp.touched[q];
         const int i = GC + c / rp.ny, j = GC + c % rp.ny;
         bool ok;
-        if constexpr (YF::kStored) {
+        if constexpr (YF::kStored && XF::kStored) {
             ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+        } else if constexpr (!XF::kStored) {  // every face from XF / YF, then cell_update's arithmetic
+            const size_t o = (size_t)i * g.ld + j;
+            double fl[4], fu[4], gl[4], gu[4], y[4], un[4], W[4], R[4];
+            XF::face(Y, F, g, ph, rp, stamp, i - 1, j, fl);
+            XF::face(Y, F, g, ph, rp, stamp, i, j, fu);
+            YF::face(Y, G, g, ph, rp, stamp, i, j - 1, gl);
+            YF::face(Y, G, g, ph, rp, stamp, i, j, gu);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                y[v] = Y[(size_t)v * g.plane + o];
+                un[v] = kind != UPD_EULER ? Un[(size_t)v * g.plane + o] : 0.0;
+            }
+            cell_update_pre(out, g, ph, dt, kind, i, j, y, un, fl, fu, gl, gu, W, R);
+            ok = admissible(Consts{ph.gamma, ph.gm1, ph.gr, Recip{}}, W);
         } else {  // the same update with the y faces from YF (cell_update's arithmetic)
             const size_t o = (size_t)i * g.ld + j;
             double gl[4], gu[4], y[4], W[4], R[4];
@@ -127,7 +146,7 @@ __device__ __forceinline__ void repair_cells_phase(const double* Un, const doubl
 // a list).  One ordinary launch per stage: with the usual empty first list it returns at
 // once, and a list of a few hundred cells takes a few microseconds.  (A cooperative
 // multi-workgroup launch with grid barriers measured 0.1 ms of stream stall per step.)
-template <typename YF>
+template <typename YF, typename XF>
 __global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const double* Y,
                                                        double* out, double* F, double* G, Geom g,
                                                        Phys ph, double dt, int kind, Repair rp,
@@ -141,7 +160,7 @@ __global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const d
         __syncthreads();
         repair_touch_phase(rp, r, stamp * 16u + (unsigned)r, 0u, 1u);
         __syncthreads();
-        repair_cells_phase<YF>(Un, Y, out, F, G, g, ph, dt, kind, rp, r, stamp, ds, faults, 0u, 1u);
+        repair_cells_phase<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, r, stamp, ds, faults, 0u, 1u);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -152,11 +171,11 @@ __global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const d
     }
 }
 
-template <typename YF>
+template <typename YF, typename XF = XFacesStored>
 void launch_repair_t(const double* Un, const double* Y, double* out, double* F, double* G,
                      const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
                      unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
-    hipLaunchKernelGGL(k_repair_block<YF>, dim3(1), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
+    hipLaunchKernelGGL((k_repair_block<YF, XF>), dim3(1), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
                        rp, stamp, ds, faults);
 }
 

@@ -1,0 +1,108 @@
+"""The mixed form's one-pass stage (x faces, y faces and the stage update in one marching
+kernel) against its two-sweep stage (x sweep, then the fused y sweep): the face and cell
+arithmetic is the same, so every state and every dt must be identical, bit for bit, over
+every solver, reconstruction, RK order, the positivity safeguard, the FP64 recompute route
+and grids narrower than one strip or one block of rows.  The two-sweep stage itself is
+checked against the FP64 form in test_gpu_mixed.py."""
+import numpy as np
+import pytest
+
+from overheatingproblem_amd import cases
+from overheatingproblem_amd.capi import Solver
+from overheatingproblem_amd.config import (DEBUG_MIXED_ALL_FP64, DEBUG_ONE_PASS, DEBUG_TWO_SWEEP,
+                                           PRECISION_MIXED)
+
+pytestmark = pytest.mark.gpu
+
+
+def run(cfg, U0, steps, flags):
+    c = cfg.copy()
+    c.precision = PRECISION_MIXED
+    c.debug_flags = c.debug_flags | flags
+    with Solver(c) as s:
+        s.set_state(U0)
+        dts = []
+        for _ in range(steps):
+            dts.append(s.get_delta_t())
+            s.time_advance(dts[-1])
+        U = s.get_state()
+        faults = s.sync()
+    return U, dts, faults
+
+
+def same_both_ways(cfg, U0, steps, flags=0):
+    a, da, fa = run(cfg, U0, steps, flags | DEBUG_ONE_PASS)
+    b, db, fb = run(cfg, U0, steps, flags | DEBUG_TWO_SWEEP)
+    assert np.isfinite(a).all()
+    assert da == db
+    assert fa == fb
+    assert np.array_equal(a, b), f"{int((a != b).sum())} entries differ"
+    return a, fa
+
+
+@pytest.mark.parametrize("scheme", [2, 1, 3, 10, 11, 22, 33])
+def test_one_pass_every_solver(scheme):
+    c, U0 = cases.lax_liu_3(200, scheme=scheme)
+    same_both_ways(c, U0, 20)
+
+
+@pytest.mark.parametrize("recon,char", [(1, 1), (2, 1), (2, 0), (3, 1), (5, 0)])
+def test_one_pass_every_reconstruction(recon, char):
+    c, U0 = cases.lax_liu_3(130, recon_order=recon, characteristic=char)
+    same_both_ways(c, U0, 12)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_one_pass_rk_orders(order):
+    c, U0 = cases.lax_liu_3(130, time_order=order)
+    same_both_ways(c, U0, 10)
+
+
+def test_one_pass_safeguard_noh():
+    """Noh with the positivity safeguard: faces deferred to the FP64 path (fault bit 4) and
+    cells repaired; the one-pass stage stores only the fluxes those fix-ups read and the
+    repair recomputes the others."""
+    c, U0 = cases.noh(256, positivity=True)
+    U, faults = same_both_ways(c, U0, 60)
+    assert faults & 4
+
+
+def test_one_pass_safeguard_sedov():
+    c, U0 = cases.sedov(200, positivity=True)
+    same_both_ways(c, U0, 30)
+
+
+def test_one_pass_fp64_route():
+    """Every face deferred: the fix-up kernels recompute all of them in FP64 (x faces listed
+    twice where two row blocks share them, y faces where two strips do)."""
+    c, U0 = cases.four_quadrant(70, 51, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3, seed=9)
+    U, faults = same_both_ways(c, U0, 3, DEBUG_MIXED_ALL_FP64)
+    assert faults & 4
+
+
+@pytest.mark.parametrize("nx,ny", [(5, 7), (37, 200), (17, 63), (16, 64), (129, 127)])
+def test_one_pass_uneven_grids(nx, ny):
+    """Grids narrower than a strip, a last strip of one cell, row counts off the 16-row
+    blocks."""
+    c, U0 = cases.four_quadrant(nx, ny, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3, seed=3)
+    same_both_ways(c, U0, 6)
+
+
+def test_default_schedule_matches_forced():
+    """The default schedule (one-pass for WENO3, two-sweep for characteristic WENO5) gives
+    the forced schedules' result."""
+    c, U0 = cases.lax_liu_3(130, recon_order=3)
+    a, _, _ = run(c, U0, 8, 0)
+    b, _, _ = run(c, U0, 8, DEBUG_TWO_SWEEP)
+    assert np.array_equal(a, b)
+    c, U0 = cases.lax_liu_3(130)
+    a, _, _ = run(c, U0, 8, 0)
+    b, _, _ = run(c, U0, 8, DEBUG_ONE_PASS)
+    assert np.array_equal(a, b)
+
+
+def test_one_pass_headline_size():
+    """Noh 4096^2 with the safeguard, 3 steps: every strip and row block of the headline
+    grid, one-pass and two-sweep identical."""
+    c, U0 = cases.noh(4096, positivity=True)
+    same_both_ways(c, U0, 3)
