@@ -370,7 +370,8 @@ static bool one_pass(const rpfvm_ctx* x) {
 // One stage: ghost fill of Y, then either the one-pass stage (x faces, y faces, residual
 // and stage combine in one kernel) or the x-face fluxes followed by the y faces fused with
 // the residual and stage combine into out (separately when out aliases Y, which the fused
-// kernel's neighbour reads forbid), then the positivity safeguard.
+// kernel's neighbour reads forbid), then the mixed form's fix-up or the positivity
+// safeguard's rounds.
 // `last`: the step's last stage, which leaves the next step's wave-speed slots (fused
 // path only; returns whether it did).
 static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
@@ -379,42 +380,34 @@ static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
     enqueue_bc(x, Y, t_stage);
     const bool fused = out != Y;
     const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
-    if (fused && one_pass(x)) {
-        {
-            Timed t(x, RPFVM_KCLASS_FLUX_Y);
-            x->flux.stage(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds,
-                          x->faults, x->stream);
-        }
-        if (x->rp.cnt) {
-            Timed t(x, RPFVM_KCLASS_UPDATE);
-            x->flux.repair1(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds,
-                            x->faults, x->stream);
-        }
-        return ds.slot != nullptr;
-    }
-    {
-        Timed t(x, RPFVM_KCLASS_FLUX_X);
-        x->flux.x(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
-    }
-    if (fused) {
+    const bool op = fused && one_pass(x);
+    if (op) {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
-        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds,
-                   x->faults, x->stream);
-    } else {
+        x->flux.stage(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+                      x->stream);
+    } else if (fused) {
         {
-            Timed t(x, RPFVM_KCLASS_FLUX_Y);
-            x->flux.y(Y, x->G, x->g, x->ph, x->faults, x->fix, x->stream);
+            Timed t(x, RPFVM_KCLASS_FLUX_X);
+            x->flux.xs(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
         }
+        Timed t(x, RPFVM_KCLASS_FLUX_Y);
+        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+                   x->stream);
+    } else {
+        enqueue_fluxes(x, Y);
         enqueue_update(x, Un, Y, out, dt, kind, x->rp);
     }
-    if (x->rp.cnt) {
+    // the mixed form's fix-up (deferred faces, their cells, the safeguard's rounds), or the
+    // safeguard's rounds alone
+    const FixupLaunch fixup = op ? x->flux.fixup1 : fused ? x->flux.fixup2 : nullptr;
+    if (fixup) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
-        // the large-array forms store every y face (the row kernel), so the repair reads G
-        // there; otherwise the form's own repair (the mixed form recomputes what it did not
-        // store, which tests compare with the stored faces)
-        const RepairLaunch repair = use_buffer_forms(x->g, x->ph) ? x->flux.repair : &launch_repair;
-        repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults,
-                      x->stream);
+        fixup(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+              x->stream);
+    } else if (x->rp.cnt) {
+        Timed t(x, RPFVM_KCLASS_UPDATE);
+        const RepairLaunch repair = x->flux.repair ? x->flux.repair : &launch_repair;
+        repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults, x->stream);
     }
     return ds.slot != nullptr;
 }
@@ -509,6 +502,7 @@ int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     enqueue_bc(x, x->U, x->time);
     enqueue_fluxes(x, x->U);
     enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, Repair{});
+    if (x->fix.cnt) launch_fix_reset(x->fix, x->stream);  // the y sweep's list is left filled
     HIPCHK(hipGetLastError());
     int e = 0;
     if (L && (e = copy_out(x, L, x->L))) return e;

@@ -886,7 +886,7 @@ __global__ __launch_bounds__(256) void k_batch_speeds_all(BatchArgs a, int pi) {
 // ------------------------------------------------------------------ fix-up
 // Mixed form: each deferred cell with its four faces recomputed (mixed arithmetic,
 // FP64 for a face whose FP32 states are not admissible) -- the single-problem mixed
-// path's k_flux_fix + k_yfix_cells, per cell.
+// path's fix-up (k_fixfaces_mx + k_fixcells_mx), per cell.
 template <int ORDER, bool CHAR>
 __global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int kind, int yi,
                                                    int oi, int last) {

@@ -170,7 +170,9 @@ __device__ __forceinline__ void cell_update_pre(double* out, const Geom& g, cons
 }
 
 // FaceOp::compute(st, f) returns false when the face must be recomputed later (mixed
-// form); FaceOp::defer(c0) records such a face.  With PREFETCH the cell's x fluxes and
+// form); FaceOp::defer(c0) records such a face.  FaceOp::kXDeferred: the x sweep may have
+// deferred x faces too, writing NaN into F for them (the mixed form); a cell next to such
+// a face is left to the fix-up like one next to a deferred y face.  With PREFETCH the cell's x fluxes and
 // U^n are loaded before the face arithmetic, so that their latency overlaps it (24 more
 // VGPRs: it pays where the kernel keeps its occupancy, the mixed form at 4 waves/SIMD,
 // and loses where it does not, the FP64 form).
@@ -180,6 +182,7 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
                                                   const Geom& g, const Phys& ph, double dt,
                                                   int kind, const Repair& rp, const DtSlots& ds,
                                                   FaceOp& op) {
+    static_assert(!FaceOp::kXDeferred || PREFETCH, "the x-face test reads the prefetched fluxes");
     const int t = threadIdx.x;
     const Tile2 b = xcd_tile();
     const int i = GC + b.y * blockDim.y + threadIdx.y;
@@ -238,7 +241,9 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
             if (sx > lx) lx = sx;
             if (sy > ly) ly = sy;
         }
-        if (rp.cnt && !pending && !pending_up) {
+        bool cpend = pending || pending_up;
+        if constexpr (FaceOp::kXDeferred) cpend = cpend || isnan(fl[0]) || isnan(fu[0]);
+        if (rp.cnt && !cpend) {
             const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
             if (!admissible(kc, W)) {
                 const unsigned k = atomicAdd(&rp.cnt[0], 1u);
@@ -393,11 +398,17 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
 #pragma unroll
         for (int v = 0; v < 4; v++) gu[v] = __shfl_down(gf[v], 1, 64);
         const int pending_up = __shfl_down(pending, 1, 64);    // face f + 1 deferred
-        const int pending_down = __shfl_up(pending, 1, 64);    // face f - 1 deferred
-        // G: every face (FaceOp::kStoreAllG), or only those the fix-up of a deferred
-        // neighbour face reads (the two cells beside face d use faces d - 1 and d + 1; the
-        // face shared by two strips is computed by both, each seeing its own neighbour)
-        if (face_ok && ok && (FaceOp::kStoreAllG || pending_up || pending_down)) {
+        // the cell is left to the fix-up when one of its faces was deferred: its y faces f,
+        // f + 1, or (FaceOp::kXDeferred) an x face, which the x sweep marked with a NaN
+        int cpend = cell_ok && (pending | pending_up) ? 1 : 0;
+        if constexpr (FaceOp::kXDeferred) {
+            if (cell_ok && (isnan(fl[0]) || isnan(frow[t]))) cpend = 1;
+        }
+        const int cpend_dn = __shfl_up(cpend, 1, 64);  // the cell below face f (lane t - 1's)
+        // G: every face (FaceOp::kStoreAllG), or only those the fix-up of a deferred cell
+        // reads (the y faces of the cells on either side; the face shared by two strips is
+        // computed by both, each seeing its own neighbour cell)
+        if (face_ok && ok && (FaceOp::kStoreAllG || cpend || cpend_dn)) {
 #pragma unroll
             for (int v = 0; v < 4; v++) buffer_store_f64(gf[v], rg, fb, (int)(rowb + v * pl8), 0);
         }
@@ -420,7 +431,7 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
                 if (sx > lx) lx = sx;
                 if (sy > ly) ly = sy;
             }
-            if (rp.cnt && !pending && !pending_up) {
+            if (rp.cnt && !cpend) {
                 const Consts kc{ph.gamma, ph.gm1, ph.gr, Recip{}};
                 if (!admissible(kc, W)) {
                     const unsigned k = atomicAdd(&rp.cnt[0], 1u);

@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ Ls,
 template <int ORDER, bool CHAR, int SOLVER>
 struct FaceOp64 {
     static constexpr bool kStoreAllG = true;  // the safeguard reads every y face from G
+    static constexpr bool kXDeferred = false;  // every x face is in F
     const Phys& ph;
     Consts kc;
     unsigned* faults;
@@ -417,7 +418,8 @@ static void launch_yflux_update(const double* Un, const double* Y, double* out, 
 template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair pick3() {
     return FluxPair{&launch_flux<1, ORDER, CHAR, SOLVER>, &launch_flux<2, ORDER, CHAR, SOLVER>,
-                    &launch_yflux_update<ORDER, CHAR, SOLVER>, &launch_repair, nullptr, nullptr};
+                    &launch_flux<1, ORDER, CHAR, SOLVER>, &launch_yflux_update<ORDER, CHAR, SOLVER>,
+                    &launch_repair, nullptr, nullptr, nullptr};
 }
 
 template <int ORDER, bool CHAR>

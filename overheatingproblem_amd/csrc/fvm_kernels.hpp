@@ -122,18 +122,23 @@ using RepairLaunch = void (*)(const double* Un, const double* Y, double* out, do
                               unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
 
 // The whole stage in one pass (fvm_onepass.hpp): x faces, y faces, residual and stage
-// combine, then the fix-ups of deferred faces (out must not alias Y; buffer forms only).
+// combine (out must not alias Y; buffer forms only).
 using StageLaunch = void (*)(const double* Un, const double* Y, double* out, double* F, double* G,
                              const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
                              const FaceFix& fix, unsigned stamp, const DtSlots& ds,
                              unsigned* faults, hipStream_t s);
+// After a stage of the mixed form: the FP64 recomputation of the deferred faces, the update
+// of their cells, the positivity safeguard's rounds; clears the lists.
+using FixupLaunch = StageLaunch;
 
 struct FluxPair {
-    FluxLaunch x, y;
-    FusedLaunch yu;
-    RepairLaunch repair;
-    StageLaunch stage;      // null: the form has no one-pass stage
-    RepairLaunch repair1;   // the safeguard's rounds after a one-pass stage
+    FluxLaunch x, y;      // complete face fluxes into F / G (space operator, unfused stages)
+    FluxLaunch xs;        // the two-sweep stage's x sweep (the mixed form defers faces to fixup2)
+    FusedLaunch yu;       // y faces fused with the residual and stage combine
+    RepairLaunch repair;  // the safeguard's rounds after a stage of a form without fix-ups
+    StageLaunch stage;    // the one-pass stage (null: the form has none)
+    FixupLaunch fixup2;   // after xs + yu (null: none)
+    FixupLaunch fixup1;   // after the one-pass stage
 };
 
 FluxPair select_flux(int order, bool characteristic, int solver);
@@ -148,11 +153,8 @@ void launch_update(const double* Un, const double* Y, double* out, const double*
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
                    const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
                    unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
-// Cells next to the y faces the mixed form recomputed in FP64 (fvm_mixed_kernels.hip).
-void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
-                       const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                       const Repair& rp, const FaceFix& fix, unsigned stamp, const DtSlots& ds,
-                       hipStream_t s);
+// Clears the mixed form's FP64 recompute lists (after the space operator's sweeps).
+void launch_fix_reset(const FaceFix& fix, hipStream_t s);
 void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s);
 int dt_blocks(const Geom& g);
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,

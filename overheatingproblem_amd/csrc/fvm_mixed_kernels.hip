@@ -1,11 +1,16 @@
 // fvm_mixed_kernels.hip -- face-flux kernels of the mixed-precision form.
 //
-// Same launch geometry and output as k_flux (fvm_kernels.hip); the face arithmetic is
-// fvm::mx (fvm_mixed.hpp).  A face whose FP32-reconstructed states are not admissible
-// (rho <= 0 or p <= 0 or not finite) is appended to a list instead, and k_flux_fix
-// recomputes the listed faces entirely with the FP64 path right after (with the
-// positivity safeguard on, a face still inadmissible there falls back to first order).
-// Keeping the rare FP64 path out of the hot kernel keeps its registers and occupancy.
+// The face arithmetic is fvm::mx (fvm_mixed.hpp).  A face whose FP32-reconstructed states
+// are not admissible (rho <= 0 or p <= 0 or not finite) is appended to a list instead of
+// computed, and its cells are left to the fix-up after the stage's sweeps: k_fixfaces_mx
+// recomputes the listed faces with the FP64 path (with the positivity safeguard on, a face
+// still inadmissible there falls back to first order), k_fixcells_mx updates their cells
+// again and runs the safeguard's rounds.  Keeping the rare FP64 path out of the hot
+// kernels keeps their registers and occupancy.
+// A stage is either two sweeps -- k_flux_mixed (x faces into F; a deferred face gets NaN
+// there) and the fused y sweep k_yflux_march_mixed, which leaves the cells next to a
+// deferred face of either direction to the fix-up -- or one pass, k_stage_mixed
+// (fvm_onepass.hpp).
 #include <hip/hip_runtime.h>
 
 #include "fvm_buffer.hpp"
@@ -73,6 +78,9 @@ __global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
     mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
     if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR))) {
         defer_face(fix, DIR - 1, c0);
+        // the fused y sweep leaves the cells of a NaN x face to the fix-up
+#pragma unroll
+        for (int v = 0; v < 4; v++) io.store(F, v, c0, __builtin_nan(""));
         return;
     }
     mx::riemann<DIR, SOLVER>(ph.gamma, UL, UR, f);
@@ -80,8 +88,9 @@ __global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
     for (int v = 0; v < 4; v++) io.store(F, v, c0, f[v]);
 }
 
-// FP64 recomputation of the listed faces; also resets the other direction's list,
-// which the next flux kernel (the other direction) refills.
+// FP64 recomputation of the listed faces of one direction, for the complete sweeps of
+// rpfvm_space_operator; also resets the other direction's list, which the next sweep (the
+// other direction) refills.
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
 __global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
                                                   double* __restrict__ F, Geom g, Phys ph,
@@ -106,13 +115,14 @@ __global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
 }
 
 // Mixed-form y faces fused with the cell update (fvm_fused.hpp).  A face that needs the
-// FP64 path is deferred to fix.list[1]; its two cells are updated again by
-// k_yfix_cells once k_flux_fix<2> has recomputed it.
+// FP64 path is deferred to fix.list[1]; the cells next to it, and those next to an x face
+// the x sweep deferred, are updated again by the fix-up.
 template <int ORDER, bool CHAR, int SOLVER>
 struct FaceOpMx {
     // The fused sweep stores a face flux in G only where the FP64 fix-up reads it (next to
-    // a deferred face); the safeguard recomputes the others (YFacesMx).
+    // a deferred cell); the safeguard recomputes the others (YFacesMx).
     static constexpr bool kStoreAllG = false;
+    static constexpr bool kXDeferred = true;
     const Phys& ph;
     FaceFix fix;
     __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
@@ -175,43 +185,6 @@ __global__ __launch_bounds__(64 * kMarchWaves, 4) void k_yflux_march_mixed(const
     yflux_update_march(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, op, lds);
 }
 
-// Cells on either side of each recomputed y face: residual + stage combine again, with
-// the final G.  A cell next to two such faces is computed twice with identical results;
-// the positivity list receives it once (stamp de-duplication).
-__global__ __launch_bounds__(256) void k_yfix_cells(const double* Un, const double* Y, double* out,
-                                                    const double* __restrict__ F,
-                                                    const double* __restrict__ G, Geom g, Phys ph,
-                                                    double dt, int kind, Repair rp, FaceFix fix,
-                                                    unsigned tstamp, DtSlots ds) {
-    const unsigned n = min(fix.cnt[1], fix.cap);
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * n; q += gridDim.x * blockDim.x) {
-        const size_t c0 = (size_t)fix.list[1][q >> 1];
-        const int i = (int)(c0 / g.ld);
-        const int j = (int)(c0 % g.ld) + (int)(q & 1);
-        if (j < GC || j > g.C - GC - 1) continue;
-        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
-        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
-            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
-            ds.slot[2 * w] = kSlotStale;
-        }
-        if (rp.cnt && !ok) {
-            const int cell = (i - GC) * rp.ny + (j - GC);
-            if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
-                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
-                rp.list[0][k] = cell;
-            }
-        }
-    }
-}
-
-void launch_yfix_cells(const double* Un, const double* Y, double* out, const double* F,
-                       const double* G, const Geom& g, const Phys& ph, double dt, int kind,
-                       const Repair& rp, const FaceFix& fix, unsigned stamp, const DtSlots& ds,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_yfix_cells, dim3(64), dim3(256), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
-                       rp, fix, stamp * 16u + 15u, ds);
-}
-
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_yflux_update_mixed(const double* Un, const double* Y, double* out,
                                       const double* F, double* G, const Geom& g, const Phys& ph,
@@ -232,12 +205,11 @@ static void launch_yflux_update_mixed(const double* Un, const double* Y, double*
         hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER, false>), yflux_update_grid(g),
                            yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
     }
-    hipLaunchKernelGGL((k_flux_fix<2, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, G, g,
-                       ph, fix, faults);
-    launch_yfix_cells(Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, s);
 }
 
-template <int DIR, int ORDER, bool CHAR, int SOLVER>
+// The x (DIR 1) or y (DIR 2) faces into F; FIX: their FP64 recomputation right after
+// (complete fluxes, rpfvm_space_operator), else left to the stage's fix-up.
+template <int DIR, int ORDER, bool CHAR, int SOLVER, bool FIX>
 static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const Phys& ph,
                               unsigned* faults, const FaceFix& fix, hipStream_t s) {
     const int nj = DIR == 1 ? g.C - 2 * GC : g.C - 2 * GC + 1;
@@ -250,14 +222,25 @@ static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const P
     else
         hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER, false>), grid, block, 0, s, Y, F, g,
                            ph, fix);
-    hipLaunchKernelGGL((k_flux_fix<DIR, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, g,
-                       ph, fix, faults);
+    if (FIX)
+        hipLaunchKernelGGL((k_flux_fix<DIR, ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, g,
+                           ph, fix, faults);
+}
+
+__global__ void k_fix_reset(FaceFix fix) {
+    fix.cnt[0] = 0;
+    fix.cnt[1] = 0;
+    fix.cnt[2] = 0;
+}
+
+void launch_fix_reset(const FaceFix& fix, hipStream_t s) {
+    hipLaunchKernelGGL(k_fix_reset, dim3(1), dim3(1), 0, s, fix);
 }
 
 // The y-face fluxes of the safeguard's re-updated cells in the mixed form, which does not
 // store them all: face (i, f) between cells (i, f) and (i, f + 1) comes from G when the
 // safeguard replaced it (a neighbour repaired in this stage) or when the FP32 path
-// deferred it (k_flux_fix<2> wrote its FP64 flux there); otherwise it is recomputed with
+// deferred it (the fix-up wrote its FP64 flux there); otherwise it is recomputed with
 // the fused sweep's own arithmetic, which gives the value the sweep had.
 template <int ORDER, bool CHAR, int SOLVER>
 struct YFacesMx {
@@ -282,20 +265,11 @@ struct YFacesMx {
     }
 };
 
-template <int ORDER, bool CHAR, int SOLVER>
-static void launch_repair_mx(const double* Un, const double* Y, double* out, double* F, double* G,
-                             const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
-                             unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
-    launch_repair_t<YFacesMx<ORDER, CHAR, SOLVER>>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds,
-                                                   faults, s);
-}
-
 // ------------------------------------------------------------- one-pass stage
 // x faces, y faces, residual and stage combine in one marching kernel (fvm_onepass.hpp):
 // the x-face fluxes never reach HBM.  Faces whose FP32 states are not admissible go to the
-// two FP64 lists; k_flux_fix_xy recomputes them and k_fix_cells_xy updates their cells
-// again.  The 13 KB LDS ring of each wavefront allows 12 per CU (3 per SIMD);
-// op_occupancy sets the register budget.
+// two FP64 lists for the fix-up.  The 13 KB LDS ring of each wavefront allows 12 per CU
+// (3 per SIMD); op_occupancy sets the register budget.
 template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
 __global__ __launch_bounds__(64 * kOpWaves, (op_occupancy<ORDER, CHAR>())) void k_stage_mixed(const double* Un, const double* Y,
                                                                   double* out, double* F, double* G,
@@ -305,78 +279,6 @@ __global__ __launch_bounds__(64 * kOpWaves, (op_occupancy<ORDER, CHAR>())) void 
     FaceOpMxX<ORDER, CHAR, SOLVER> opx{ph, fix};
     FaceOpMx<ORDER, CHAR, SOLVER> opy{ph, fix};
     stage_march(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, opx, opy, lds);
-}
-
-// FP64 recomputation of both lists' faces (x faces into F, y faces into G).
-template <int ORDER, bool CHAR, int SOLVER>
-__global__ __launch_bounds__(256) void k_flux_fix_xy(const double* __restrict__ Y, double* F,
-                                                     double* G, Geom g, Phys ph, FaceFix fix,
-                                                     unsigned* __restrict__ faults) {
-    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (n0 | n1)) atomicOr(faults, 4u);
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n0 + n1; q += gridDim.x * blockDim.x) {
-        const bool xf = q < n0;
-        const size_t c0 = (size_t)(xf ? fix.list[0][q] : fix.list[1][q - n0]);
-        const size_t step = xf ? (size_t)g.ld : 1;
-        double st[6][4], f[4];
-#pragma unroll
-        for (int m = 0; m < 6; m++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (m - 2) * step];
-        const int bad = xf ? face_fp64<1, ORDER, CHAR, SOLVER>(ph, st, f)
-                           : face_fp64<2, ORDER, CHAR, SOLVER>(ph, st, f);
-        if (bad) atomicOr(faults, 1u);
-        double* D = xf ? F : G;
-#pragma unroll
-        for (int v = 0; v < 4; v++) D[(size_t)v * g.plane + c0] = f[v];
-    }
-}
-
-// The two cells of every listed face, updated again from F and G (cell_update); then the
-// last workgroup to finish clears both lists for the next stage.
-__global__ __launch_bounds__(256) void k_fix_cells_xy(const double* Un, const double* Y, double* out,
-                                                      const double* __restrict__ F,
-                                                      const double* __restrict__ G, Geom g, Phys ph,
-                                                      double dt, int kind, Repair rp, FaceFix fix,
-                                                      unsigned tstamp, DtSlots ds) {
-    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
-    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (n0 + n1); q += gridDim.x * blockDim.x) {
-        int i, j;
-        if (q < 2 * n0) {  // x face (i, j): cells (i, j) and (i + 1, j)
-            const size_t c0 = (size_t)fix.list[0][q >> 1];
-            i = (int)(c0 / g.ld) + (int)(q & 1);
-            j = (int)(c0 % g.ld);
-            if (i < GC || i > g.R - GC - 1) continue;
-        } else {  // y face (i, j): cells (i, j) and (i, j + 1)
-            const unsigned r = q - 2 * n0;
-            const size_t c0 = (size_t)fix.list[1][r >> 1];
-            i = (int)(c0 / g.ld);
-            j = (int)(c0 % g.ld) + (int)(r & 1);
-            if (j < GC || j > g.C - GC - 1) continue;
-        }
-        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
-        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
-            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
-            ds.slot[2 * w] = kSlotStale;
-        }
-        if (rp.cnt && !ok) {
-            const int cell = (i - GC) * rp.ny + (j - GC);
-            if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
-                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
-                rp.list[0][k] = cell;
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&fix.cnt[2], 1u) == gridDim.x - 1) {  // every workgroup has read the counts
-            fix.cnt[0] = 0;
-            fix.cnt[1] = 0;
-            fix.cnt[2] = 0;
-            __threadfence();
-        }
-    }
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
@@ -390,10 +292,6 @@ static void launch_stage_mixed(const double* Un, const double* Y, double* out, d
     else
         hipLaunchKernelGGL((k_stage_mixed<ORDER, CHAR, SOLVER, false>), stage_march_grid(g),
                            stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
-    hipLaunchKernelGGL((k_flux_fix_xy<ORDER, CHAR, SOLVER>), dim3(64), dim3(256), 0, s, Y, F, G, g, ph,
-                       fix, faults);
-    hipLaunchKernelGGL(k_fix_cells_xy, dim3(64), dim3(256), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp,
-                       fix, stamp * 16u + 15u, ds);
 }
 
 // The x-face fluxes of the safeguard's re-updated cells after a one-pass stage, which stores
@@ -423,22 +321,136 @@ struct XFacesMx {
     }
 };
 
+// ------------------------------------------------------------------ fix-up
+// Two launches after a stage's sweeps.  k_fixfaces_mx recomputes the listed faces with the
+// FP64 path (grid-stride: x faces into F, y faces into G).  k_fixcells_mx updates the two
+// cells of every listed face again from F and G (cell_update; a cell next to two listed
+// faces is computed twice with identical results and enters the safeguard's list once);
+// its last workgroup to finish runs the safeguard's rounds and clears the lists.  YF / XF:
+// where the rounds take the faces of a re-updated cell (fvm_repair.hpp).
+constexpr int kFixFaceBlocks = 256, kFixCellBlocks = 64;
+
 template <int ORDER, bool CHAR, int SOLVER>
-static void launch_repair1_mx(const double* Un, const double* Y, double* out, double* F, double* G,
-                              const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
-                              unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
-    launch_repair_t<YFacesMx<ORDER, CHAR, SOLVER>, XFacesMx<ORDER, CHAR, SOLVER>>(
-        Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults, s);
+__global__ __launch_bounds__(256) void k_fixfaces_mx(const double* __restrict__ Y, double* F, double* G,
+                                                     Geom g, Phys ph, FaceFix fix,
+                                                     unsigned* __restrict__ faults) {
+    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n0 | n1)) atomicOr(faults, 4u);
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n0 + n1; q += gridDim.x * blockDim.x) {
+        const bool xf = q < n0;
+        const size_t c0 = (size_t)(xf ? fix.list[0][q] : fix.list[1][q - n0]);
+        const size_t step = xf ? (size_t)g.ld : 1;
+        double st[6][4], f[4];
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (m - 2) * step];
+        const int bad = xf ? face_fp64<1, ORDER, CHAR, SOLVER>(ph, st, f)
+                           : face_fp64<2, ORDER, CHAR, SOLVER>(ph, st, f);
+        if (bad) atomicOr(faults, 1u);
+        double* D = xf ? F : G;
+#pragma unroll
+        for (int v = 0; v < 4; v++) D[(size_t)v * g.plane + c0] = f[v];
+    }
+}
+
+template <typename YF, typename XF>
+__global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const double* Y, double* out,
+                                                      double* F, double* G, Geom g, Phys ph, double dt,
+                                                      int kind, Repair rp, FaceFix fix, unsigned stamp,
+                                                      DtSlots ds, unsigned* __restrict__ faults) {
+    const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
+    const unsigned tstamp = stamp * 16u + 15u;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (n0 + n1); q += gridDim.x * blockDim.x) {
+        int i, j;
+        if (q < 2 * n0) {  // x face (i, j): cells (i, j) and (i + 1, j)
+            const size_t c0 = (size_t)fix.list[0][q >> 1];
+            i = (int)(c0 / g.ld) + (int)(q & 1);
+            j = (int)(c0 % g.ld);
+            if (i < GC || i > g.R - GC - 1) continue;
+        } else {  // y face (i, j): cells (i, j) and (i, j + 1)
+            const unsigned r = q - 2 * n0;
+            const size_t c0 = (size_t)fix.list[1][r >> 1];
+            i = (int)(c0 / g.ld);
+            j = (int)(c0 % g.ld) + (int)(r & 1);
+            if (j < GC || j > g.C - GC - 1) continue;
+        }
+        const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+        if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
+            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
+            ds.slot[2 * w] = kSlotStale;
+        }
+        if (rp.cnt && !ok) {
+            const int cell = (i - GC) * rp.ny + (j - GC);
+            if (atomicExch(&rp.tch[cell], tstamp) != tstamp) {
+                const unsigned k = atomicAdd(&rp.cnt[0], 1u);
+                rp.list[0][k] = cell;
+            }
+        }
+    }
+    // every thread publishes its cells; the workgroup that counts last goes on
+    __threadfence();
+    __syncthreads();
+    __shared__ int last;
+    if (threadIdx.x == 0) last = atomicAdd(&fix.cnt[2], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (rp.cnt) repair_rounds<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults);
+    if (threadIdx.x == 0) {
+        fix.cnt[0] = 0;
+        fix.cnt[1] = 0;
+        fix.cnt[2] = 0;
+    }
+}
+
+template <int ORDER, bool CHAR, int SOLVER, typename YF, typename XF>
+static void launch_fixup(const double* Un, const double* Y, double* out, double* F, double* G,
+                         const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                         const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((k_fixfaces_mx<ORDER, CHAR, SOLVER>), dim3(kFixFaceBlocks), dim3(256), 0, s, Y, F,
+                       G, g, ph, fix, faults);
+    hipLaunchKernelGGL((k_fixcells_mx<YF, XF>), dim3(kFixCellBlocks), dim3(1024), 0, s, Un, Y, out, F, G,
+                       g, ph, dt, kind, rp, fix, stamp, ds, faults);
+}
+
+// After the two-sweep stage: every x face is in F (the fix-up writes the deferred ones);
+// the marching y sweep stores only the y faces the fix-up reads, the row form (arrays of
+// 4 GB and more) every one.
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_fixup2(const double* Un, const double* Y, double* out, double* F, double* G,
+                          const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                          const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
+                          hipStream_t s) {
+    if (use_buffer_forms(g, ph))
+        launch_fixup<ORDER, CHAR, SOLVER, YFacesMx<ORDER, CHAR, SOLVER>, XFacesStored>(
+            Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, faults, s);
+    else
+        launch_fixup<ORDER, CHAR, SOLVER, YFacesStored, XFacesStored>(Un, Y, out, F, G, g, ph, dt, kind,
+                                                                      rp, fix, stamp, ds, faults, s);
+}
+
+// After the one-pass stage, which stores only the faces of either direction the fix-up reads.
+template <int ORDER, bool CHAR, int SOLVER>
+static void launch_fixup1(const double* Un, const double* Y, double* out, double* F, double* G,
+                          const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                          const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
+                          hipStream_t s) {
+    launch_fixup<ORDER, CHAR, SOLVER, YFacesMx<ORDER, CHAR, SOLVER>, XFacesMx<ORDER, CHAR, SOLVER>>(
+        Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, faults, s);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
 static FluxPair mpick3() {
-    return FluxPair{&launch_flux_mixed<1, ORDER, CHAR, SOLVER>,
-                    &launch_flux_mixed<2, ORDER, CHAR, SOLVER>,
+    return FluxPair{&launch_flux_mixed<1, ORDER, CHAR, SOLVER, true>,
+                    &launch_flux_mixed<2, ORDER, CHAR, SOLVER, true>,
+                    &launch_flux_mixed<1, ORDER, CHAR, SOLVER, false>,
                     &launch_yflux_update_mixed<ORDER, CHAR, SOLVER>,
-                    &launch_repair_mx<ORDER, CHAR, SOLVER>,
+                    nullptr,
                     &launch_stage_mixed<ORDER, CHAR, SOLVER>,
-                    &launch_repair1_mx<ORDER, CHAR, SOLVER>};
+                    &launch_fixup2<ORDER, CHAR, SOLVER>,
+                    &launch_fixup1<ORDER, CHAR, SOLVER>};
 }
 
 template <int ORDER, bool CHAR>

@@ -146,12 +146,12 @@ __device__ __forceinline__ void repair_cells_phase(const double* Un, const doubl
 // a list).  One ordinary launch per stage: with the usual empty first list it returns at
 // once, and a list of a few hundred cells takes a few microseconds.  (A cooperative
 // multi-workgroup launch with grid barriers measured 0.1 ms of stream stall per step.)
+// The rounds, run by every thread of one workgroup (any size).
 template <typename YF, typename XF>
-__global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const double* Y,
-                                                       double* out, double* F, double* G, Geom g,
-                                                       Phys ph, double dt, int kind, Repair rp,
-                                                       unsigned stamp, DtSlots ds,
-                                                       unsigned* __restrict__ faults) {
+__device__ __forceinline__ void repair_rounds(const double* Un, const double* Y, double* out, double* F,
+                                              double* G, const Geom& g, const Phys& ph, double dt,
+                                              int kind, const Repair& rp, unsigned stamp,
+                                              const DtSlots& ds, unsigned* __restrict__ faults) {
     if (__hip_atomic_load(&rp.cnt[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
     int r = 0;
     for (; r < kRepairRounds; r++) {
@@ -169,6 +169,15 @@ __global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const d
         rp.cnt[1] = 0;
         rp.cnt[2] = 0;
     }
+}
+
+template <typename YF, typename XF>
+__global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const double* Y,
+                                                       double* out, double* F, double* G, Geom g,
+                                                       Phys ph, double dt, int kind, Repair rp,
+                                                       unsigned stamp, DtSlots ds,
+                                                       unsigned* __restrict__ faults) {
+    repair_rounds<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults);
 }
 
 template <typename YF, typename XF = XFacesStored>
