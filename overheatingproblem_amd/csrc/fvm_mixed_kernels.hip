@@ -5,7 +5,7 @@
 // computed, and its cells are left to the fix-up after the stage's sweeps: k_fixfaces_mx
 // recomputes the listed faces with the FP64 path (with the positivity safeguard on, a face
 // still inadmissible there falls back to first order), k_fixcells_mx updates their cells
-// again and runs the safeguard's rounds.  Keeping the rare FP64 path out of the hot
+// again, and the safeguard's rounds follow.  Keeping the rare FP64 path out of the hot
 // kernels keeps their registers and occupancy.
 // A stage is either two sweeps -- k_flux_mixed (x faces into F; a deferred face gets NaN
 // there) and the fused y sweep k_yflux_march_mixed, which leaves the cells next to a
@@ -322,12 +322,12 @@ struct XFacesMx {
 };
 
 // ------------------------------------------------------------------ fix-up
-// Two launches after a stage's sweeps.  k_fixfaces_mx recomputes the listed faces with the
-// FP64 path (grid-stride: x faces into F, y faces into G).  k_fixcells_mx updates the two
-// cells of every listed face again from F and G (cell_update; a cell next to two listed
-// faces is computed twice with identical results and enters the safeguard's list once);
-// its last workgroup to finish runs the safeguard's rounds and clears the lists.  YF / XF:
-// where the rounds take the faces of a re-updated cell (fvm_repair.hpp).
+// After a stage's sweeps.  k_fixfaces_mx recomputes the listed faces with the FP64 path
+// (grid-stride: x faces into F, y faces into G).  k_fixcells_mx updates the two cells of
+// every listed face again from F and G (cell_update; a cell next to two listed faces is
+// computed twice with identical results and enters the safeguard's list once) and clears
+// the lists.  Then the safeguard's rounds (k_repair_block); YF / XF: where they take the
+// faces of a re-updated cell (fvm_repair.hpp).
 constexpr int kFixFaceBlocks = 256, kFixCellBlocks = 64;
 
 template <int ORDER, bool CHAR, int SOLVER>
@@ -354,11 +354,11 @@ __global__ __launch_bounds__(256) void k_fixfaces_mx(const double* __restrict__ 
     }
 }
 
-template <typename YF, typename XF>
 __global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const double* Y, double* out,
-                                                      double* F, double* G, Geom g, Phys ph, double dt,
-                                                      int kind, Repair rp, FaceFix fix, unsigned stamp,
-                                                      DtSlots ds, unsigned* __restrict__ faults) {
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ G, Geom g, Phys ph,
+                                                      double dt, int kind, Repair rp, FaceFix fix,
+                                                      unsigned stamp, DtSlots ds) {
     const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
     const unsigned tstamp = stamp * 16u + 15u;
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (n0 + n1); q += gridDim.x * blockDim.x) {
@@ -388,19 +388,14 @@ __global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const do
             }
         }
     }
-    // every thread publishes its cells; the workgroup that counts last goes on
-    __threadfence();
+    // The last workgroup to count clears the lists: every workgroup read both counts before
+    // counting itself, and only the counters are shared (no data to publish, so no fence,
+    // which on this multi-XCD part would write an L2 back).
     __syncthreads();
-    __shared__ int last;
-    if (threadIdx.x == 0) last = atomicAdd(&fix.cnt[2], 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (rp.cnt) repair_rounds<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults);
-    if (threadIdx.x == 0) {
-        fix.cnt[0] = 0;
-        fix.cnt[1] = 0;
-        fix.cnt[2] = 0;
+    if (threadIdx.x == 0 && atomicAdd(&fix.cnt[2], 1u) == gridDim.x - 1) {
+        atomicExch(&fix.cnt[0], 0u);
+        atomicExch(&fix.cnt[1], 0u);
+        atomicExch(&fix.cnt[2], 0u);
     }
 }
 
@@ -411,8 +406,9 @@ static void launch_fixup(const double* Un, const double* Y, double* out, double*
                          hipStream_t s) {
     hipLaunchKernelGGL((k_fixfaces_mx<ORDER, CHAR, SOLVER>), dim3(kFixFaceBlocks), dim3(256), 0, s, Y, F,
                        G, g, ph, fix, faults);
-    hipLaunchKernelGGL((k_fixcells_mx<YF, XF>), dim3(kFixCellBlocks), dim3(1024), 0, s, Un, Y, out, F, G,
-                       g, ph, dt, kind, rp, fix, stamp, ds, faults);
+    hipLaunchKernelGGL(k_fixcells_mx, dim3(kFixCellBlocks), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, dt,
+                       kind, rp, fix, stamp, ds);
+    if (rp.cnt) launch_repair_t<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults, s);
 }
 
 // After the two-sweep stage: every x face is in F (the fix-up writes the deferred ones);
