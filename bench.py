@@ -47,6 +47,17 @@ KERNELS = {"fp64": {"flux_x": "k_flux_ring", "flux_y": "k_yflux_march", "update"
                      "update": "k_update", "dt": "k_dt_slots", "bc": "k_bc_side"}}
 
 
+def full_length_record(n):
+    """The whole-run accuracy record of the workload (profiles/, written by --noh-full):
+    mixed vs FP64 L1(rho) at t = 0.6, next to FP64 started from a 1e-12 perturbation, over
+    the whole grid and with the origin corner, wall bands and shock band masked."""
+    from overheatingproblem_amd.profiling import load_traffic
+    rec = load_traffic(f"noh{n}").get("accuracy_full")
+    if not rec:
+        return None
+    return dict(rec, source=f"profiles/noh{n}_t0.6_accuracy.md (python bench.py --noh-full)")
+
+
 def kernel_io_bytes(cls, nx, ny, stores_g=True):
     """Bytes one launch of a timing class must read and write (FP64, 4 vars), averaged
     over the three RK3 stages.  x sweep: the stage state and the x-face fluxes.  Fused
@@ -134,39 +145,57 @@ def host_info(cores):
                     else "every CPU of the affinity mask")}
 
 
-def run_gpu(cfg, U0, steps, warmup, dts=None):
-    """Warm-up + timed steps with dt from the device reduction (or replay `dts`, untimed
-    use).  Returns timings, kernel times, fault bits, final state and the dts used."""
+def run_gpu(cfg, U0, steps, warmup, dts=None, timing=True):
+    """Warm-up + timed steps through the device-side loop (rpfvm_advance_to: dt and clock on
+    the device, no host round trip per step), bracketed by HIP events on the context
+    stream, with the library's per-kernel events on (`timing`).  With `dts`: the
+    host-driven loop instead (rpfvm_time_advance with the given dts: the replay).  Returns
+    timings, kernel times, fault bits and the final state."""
     from overheatingproblem_amd.capi import Solver
     ev = HipEvents()
-    used = []
     with Solver(cfg) as s:
         s.set_state(U0)
-
-        def step(k):
-            dt = dts[k] if dts is not None else s.get_delta_t()
-            used.append(dt)
-            s.time_advance(dt)
-
-        for k in range(warmup):
-            step(k)
+        if dts is not None:
+            for dt in dts[:warmup]:
+                s.time_advance(dt)
+        else:
+            s.advance_to(float("inf"), warmup)
         faults = s.sync()
         stream = s.stream()
-        s.set_kernel_timing(True)
+        s.set_kernel_timing(timing)
         s.kernel_times()  # reset
         e0, e1 = ev.create(), ev.create()
         ev.record(e0, stream)
         t0 = time.perf_counter()
-        for k in range(warmup, warmup + steps):
-            step(k)
+        if dts is not None:
+            for dt in dts[warmup:warmup + steps]:
+                s.time_advance(dt)
+            n = len(dts[warmup:warmup + steps])
+        else:
+            _, n = s.advance_to(float("inf"), steps)
         ev.record(e1, stream)
         gpu_ms = ev.elapsed_ms(e0, e1)
         t1 = time.perf_counter()
         kt = s.kernel_times()
         faults |= s.sync()
         U = s.get_state()
-    return {"gpu_ms": gpu_ms, "wall_s": t1 - t0, "kernels": kt, "faults": faults, "U": U,
-            "dts": used}
+    assert n == steps
+    return {"gpu_ms": gpu_ms, "wall_s": t1 - t0, "kernels": kt, "faults": faults, "U": U}
+
+
+def host_loop_dts(cfg, U0, steps):
+    """The host-driven loop (rpfvm_get_delta_t, then rpfvm_time_advance), untimed: the dts
+    and the final state, for the mixed form's replay and as a check of the device loop."""
+    from overheatingproblem_amd.capi import Solver
+    dts = []
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        for _ in range(steps):
+            dts.append(s.get_delta_t())
+            s.time_advance(dts[-1])
+        U = s.get_state()
+        s.sync()
+    return dts, U
 
 
 def cpu_baseline(n, steps, cores):
@@ -305,9 +334,10 @@ def run_dmr(steps, warmup):
     mx = cfg.copy()
     mx.precision = PRECISION_MIXED
     cells = cfg.nx * cfg.ny
-    r64 = run_gpu(cfg, U0, steps, warmup)
-    rmx = run_gpu(mx, U0, steps, warmup)
-    rep = run_gpu(mx, U0, steps, warmup, dts=r64["dts"])
+    r64 = run_gpu(cfg, U0, steps, warmup, timing=False)
+    rmx = run_gpu(mx, U0, steps, warmup, timing=False)
+    dts, _ = host_loop_dts(cfg, U0, warmup + steps)
+    rep = run_gpu(mx, U0, steps, warmup, dts=dts, timing=False)
     p64 = cases.cons_to_prim(r64["U"][:, 4:-4, 4:-4], cfg.gamma)
     pmx = cases.cons_to_prim(rep["U"][:, 4:-4, 4:-4], cfg.gamma)
 
@@ -693,11 +723,19 @@ def main():
     cfg_mx.precision = PRECISION_MIXED
     cells = n * n
     log(f"bench: noh {n}x{n} warmup={args.warmup} steps={args.steps}: fp64 ...")
-    r64 = run_gpu(cfg, U0, args.steps, args.warmup)
+    # the throughput runs record no per-kernel events (each event in the stream costs GPU
+    # time); a second timed run of the same steps takes the per-kernel durations
+    r64 = run_gpu(cfg, U0, args.steps, args.warmup, timing=False)
+    r64["kernels"] = run_gpu(cfg, U0, args.steps, args.warmup)["kernels"]
     log(f"bench: fp64 {r64['gpu_ms'] / args.steps:.3f} ms/step; mixed ...")
-    rmx = run_gpu(cfg_mx, U0, args.steps, args.warmup)
-    log(f"bench: mixed {rmx['gpu_ms'] / args.steps:.3f} ms/step; mixed replay on fp64 dts ...")
-    rep = run_gpu(cfg_mx, U0, args.steps, args.warmup, dts=r64["dts"])
+    rmx = run_gpu(cfg_mx, U0, args.steps, args.warmup, timing=False)
+    rkt = run_gpu(cfg_mx, U0, args.steps, args.warmup)
+    rmx["kernels"] = rkt["kernels"]
+    log(f"bench: mixed {rmx['gpu_ms'] / args.steps:.3f} ms/step ({rkt['gpu_ms'] / args.steps:.3f} "
+        f"with per-kernel events); host-driven fp64 dts ...")
+    dts64, U64h = host_loop_dts(cfg, U0, args.warmup + args.steps)
+    log("bench: mixed replay on the fp64 dts (host-driven loop) ...")
+    rep = run_gpu(cfg_mx, U0, args.steps, args.warmup, dts=dts64, timing=False)
 
     def rate(r):
         return cells * args.steps / (r["gpu_ms"] / 1e3) / 1e6
@@ -791,7 +829,14 @@ def main():
                   "speedup_mixed_vs_fp64": round(value / value64, 4)},
         "accuracy": {"l1_rho_mixed_vs_fp64": l1, "linf_rho_mixed_vs_fp64": linf,
                      "l1_p_mixed_vs_fp64": l1p, "linf_p_mixed_vs_fp64": linfp,
-                     "note": f"mixed replayed on the fp64 run's {len(r64['dts'])} dts"},
+                     "note": f"mixed replayed on the fp64 run's {len(dts64)} dts",
+                     "full_length": full_length_record(n)},
+        "kernel_timing": {"source": "a second timed run of the same steps with the library's "
+                                    "per-kernel HIP events on the context stream",
+                          "mixed_ms_per_step_with_events": round(rkt["gpu_ms"] / args.steps, 4)},
+        "loop": {"kind": "device (rpfvm_advance_to: dt, clipping and clock on the device)",
+                 "fp64_matches_host_driven_loop": bool(np.array_equal(r64["U"], U64h)),
+                 "host_driven_mixed_ms_per_step": round(rep["gpu_ms"] / args.steps, 4)},
         "wall_s": round(rmx["wall_s"], 4),
     }
     if not args.no_dmr:

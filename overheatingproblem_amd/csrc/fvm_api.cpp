@@ -7,6 +7,8 @@
 // (include/CFD_convection.h:43-45).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -61,6 +63,8 @@ struct rpfvm_ctx {
     DtSlots slots{};               // last stage's wave-speed maxima (folded CFL reduction)
     bool slots_valid = false;      // the slots describe the current state U
     double* scal = nullptr;        // [0] dt, [1] t, [2] t_stop
+    StepCtl* ctl = nullptr;        // the step's control block (dt, stage clock, active)
+    StepCtl* host_ctl = nullptr;   // pinned mirror of ctl (the device loop's result)
     unsigned* faults = nullptr;
     double* host_scal = nullptr;   // pinned mirror of scal
     unsigned* host_faults = nullptr;
@@ -233,7 +237,8 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     const size_t face_bytes = ((fix_cap * 4 + 255) / 256) * 256;
     const size_t fix_bytes = c.precision == RPFVM_PRECISION_MIXED ? 2 * face_bytes + 256 : 0;
     const size_t slot_bytes = (((size_t)dt_slot_count(x->g) * 2 * sizeof(unsigned long long) + 255) / 256) * 256;
-    const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes + slot_bytes;
+    const size_t ctl_bytes = 256;
+    const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes + slot_bytes + ctl_bytes;
     he = hipMalloc(&x->pool, total);
     if (he != hipSuccess)
         return cleanup(fail(RPFVM_ERR_NOMEM, "hipMalloc(%zu): %s", total, hipGetErrorString(he)));
@@ -264,12 +269,15 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
         x->fix.cap = (unsigned)fix_cap;
     }
     x->slots.slot = reinterpret_cast<unsigned long long*>(q + small + rep_bytes + fix_bytes);
+    x->ctl = reinterpret_cast<StepCtl*>(q + small + rep_bytes + fix_bytes + slot_bytes);
+    x->ph.step = x->ctl;
     x->slots.nstrips = (c.ny + kStrip - 1) / kStrip;
     he = hipMemsetAsync(x->pool, 0, total, x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
     he = hipHostMalloc(reinterpret_cast<void**>(&x->host_scal), 256, hipHostMallocDefault);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipHostMalloc: %s", hipGetErrorString(he)));
     x->host_faults = reinterpret_cast<unsigned*>(x->host_scal + 8);
+    x->host_ctl = reinterpret_cast<StepCtl*>(x->host_scal + 16);
     he = hipStreamSynchronize(x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "sync: %s", hipGetErrorString(he)));
     *out = x;
@@ -318,12 +326,13 @@ int rpfvm_get_state(rpfvm_ctx* x, double* U) {
     return copy_out(x, U, x->U);
 }
 
-// Ghost fill of Y at clock t (read only by the double-Mach-reflection sides).
-static void enqueue_bc(rpfvm_ctx* x, double* Y, double t) {
+// Ghost fill of Y at the stage clock t0 + c dt of the step's control block (read only by
+// the double-Mach-reflection sides).
+static void enqueue_bc(rpfvm_ctx* x, double* Y, double c) {
     Timed tm(x, RPFVM_KCLASS_BC);
     for (int s = 0; s < 4; s++) {
-        x->bc[s].t = t;
-        launch_bc_side(Y, x->g, x->bc[s], x->stream);
+        x->bc[s].c = c;
+        launch_bc_side(Y, x->g, x->bc[s], x->ctl, x->stream);
     }
 }
 
@@ -339,9 +348,9 @@ static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
 }
 
 static void enqueue_update(rpfvm_ctx* x, const double* Un, const double* Y, double* out,
-                           double dt, int kind, const Repair& rp) {
+                           int kind, const Repair& rp) {
     Timed t(x, RPFVM_KCLASS_UPDATE);
-    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, rp, x->stream);
+    launch_update(Un, Y, out, x->F, x->G, x->g, x->ph, kind, rp, x->stream);
 }
 
 static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
@@ -374,16 +383,16 @@ static bool one_pass(const rpfvm_ctx* x) {
 // safeguard's rounds.
 // `last`: the step's last stage, which leaves the next step's wave-speed slots (fused
 // path only; returns whether it did).
-static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, double dt,
-                          int kind, double t_stage, bool last) {
+static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out, int kind,
+                          double c, bool last) {
     const unsigned stamp = ++x->stamp;
-    enqueue_bc(x, Y, t_stage);
+    enqueue_bc(x, Y, c);
     const bool fused = out != Y;
     const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
     const bool op = fused && one_pass(x);
     if (op) {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
-        x->flux.stage(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+        x->flux.stage(Un, Y, out, x->F, x->G, x->g, x->ph, kind, x->rp, x->fix, stamp, ds, x->faults,
                       x->stream);
     } else if (fused) {
         {
@@ -391,57 +400,62 @@ static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
             x->flux.xs(Y, x->F, x->g, x->ph, x->faults, x->fix, x->stream);
         }
         Timed t(x, RPFVM_KCLASS_FLUX_Y);
-        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+        x->flux.yu(Un, Y, out, x->F, x->G, x->g, x->ph, kind, x->rp, x->fix, stamp, ds, x->faults,
                    x->stream);
     } else {
         enqueue_fluxes(x, Y);
-        enqueue_update(x, Un, Y, out, dt, kind, x->rp);
+        enqueue_update(x, Un, Y, out, kind, x->rp);
     }
     // the mixed form's fix-up (deferred faces, their cells, the safeguard's rounds), or the
     // safeguard's rounds alone
     const FixupLaunch fixup = op ? x->flux.fixup1 : fused ? x->flux.fixup2 : nullptr;
     if (fixup) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
-        fixup(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, x->fix, stamp, ds, x->faults,
+        fixup(Un, Y, out, x->F, x->G, x->g, x->ph, kind, x->rp, x->fix, stamp, ds, x->faults,
               x->stream);
     } else if (x->rp.cnt) {
         Timed t(x, RPFVM_KCLASS_UPDATE);
         const RepairLaunch repair = x->flux.repair ? x->flux.repair : &launch_repair;
-        repair(Un, Y, out, x->F, x->G, x->g, x->ph, dt, kind, x->rp, stamp, ds, x->faults, x->stream);
+        repair(Un, Y, out, x->F, x->G, x->g, x->ph, kind, x->rp, stamp, ds, x->faults, x->stream);
     }
     return ds.slot != nullptr;
 }
 
-static int enqueue_step(rpfvm_ctx* x, double dt) {
+// The stages of one step, with dt and the stage clock from the step's control block.
+static void enqueue_stages(rpfvm_ctx* x) {
     // With the positivity safeguard or the mixed form the last stage cannot update U in
     // place (a repaired or FP64-recomputed cell is recomputed from U^n): it writes U3,
     // which takes over U's ghost frame (the reference leaves U's ghosts as filled at
     // stage 1) and becomes the state.
     double* last = x->U3 ? x->U3 : x->U;
     // stage k fills its ghosts at t + c_k dt (SSP-RK3: c = 0, 1, 1/2; RK2: 0, 1)
-    const double t = x->time;
     bool slots = false;
     switch (x->cfg.time_order) {
         case 1:
-            slots = enqueue_stage(x, x->U, x->U, last, dt, UPD_EULER, t, true);
+            slots = enqueue_stage(x, x->U, x->U, last, UPD_EULER, 0.0, true);
             break;
         case 2:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t, false);
-            slots = enqueue_stage(x, x->U, x->U1, last, dt, UPD_RK2_2, t + dt, true);
+            enqueue_stage(x, x->U, x->U, x->U1, UPD_EULER, 0.0, false);
+            slots = enqueue_stage(x, x->U, x->U1, last, UPD_RK2_2, 1.0, true);
             break;
         default:
-            enqueue_stage(x, x->U, x->U, x->U1, dt, UPD_EULER, t, false);
-            enqueue_stage(x, x->U, x->U1, x->U2, dt, UPD_RK3_2, t + dt, false);
-            slots = enqueue_stage(x, x->U, x->U2, last, dt, UPD_RK3_3, t + 0.5 * dt, true);
+            enqueue_stage(x, x->U, x->U, x->U1, UPD_EULER, 0.0, false);
+            enqueue_stage(x, x->U, x->U1, x->U2, UPD_RK3_2, 1.0, false);
+            slots = enqueue_stage(x, x->U, x->U2, last, UPD_RK3_3, 0.5, true);
             break;
     }
     x->slots_valid = slots;
-    x->time = t + dt;
     if (last != x->U) {
-        launch_copy_frame(x->U, last, x->g, x->stream);
+        launch_copy_frame(x->U, last, x->g, x->ctl, x->stream);
         x->U3 = x->U;
         x->U = last;
     }
+}
+
+static int enqueue_step(rpfvm_ctx* x, double dt) {
+    launch_step_set(x->ctl, dt, x->time, x->stream);
+    enqueue_stages(x);
+    x->time = x->time + dt;
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -453,7 +467,8 @@ int rpfvm_time_advance(rpfvm_ctx* x, double dt) {
 
 int rpfvm_boundary_conditions(rpfvm_ctx* x) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
-    enqueue_bc(x, x->U, x->time);
+    launch_step_set(x->ctl, 0.0, x->time, x->stream);
+    enqueue_bc(x, x->U, 0.0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -499,9 +514,10 @@ int rpfvm_riemann(rpfvm_ctx* x, int dir, const double* Ls, const double* Rs, dou
 
 int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     if (!x) return fail(RPFVM_ERR_INVALID, "null context");
-    enqueue_bc(x, x->U, x->time);
+    launch_step_set(x->ctl, 0.0, x->time, x->stream);
+    enqueue_bc(x, x->U, 0.0);
     enqueue_fluxes(x, x->U);
-    enqueue_update(x, x->U, x->U, x->L, 0.0, UPD_RESIDUAL, Repair{});
+    enqueue_update(x, x->U, x->U, x->L, UPD_RESIDUAL, Repair{});
     if (x->fix.cnt) launch_fix_reset(x->fix, x->stream);  // the y sweep's list is left filled
     HIPCHK(hipGetLastError());
     int e = 0;
@@ -512,31 +528,46 @@ int rpfvm_space_operator(rpfvm_ctx* x, double* L, double* F, double* G) {
     return 0;
 }
 
+// The driver loop of src/RP_FVM.c:179-191 (next_output_time == Tmax == t_stop) on the
+// device: each step is the CFL reduction deciding the step (k_dt_loop: stop at t_stop,
+// clip, stop when dt does not advance the clock) followed by the step's kernels, which do
+// nothing once the loop has stopped.  No dt crosses to the host: steps are enqueued in
+// chunks and the host reads the device clock once per chunk.  The chunk grows from 4 as
+// the remaining time allows, so at most a few steps past the end are enqueued as no-ops.
 int rpfvm_advance_to(rpfvm_ctx* x, double t_stop, int max_steps, double* t, int* steps) {
     if (!x || !t) return fail(RPFVM_ERR_INVALID, "null argument");
-    double time = *t;
-    x->time = time;
+    x->time = *t;
     int n = 0;
-    // src/RP_FVM.c:179-191 with next_output_time == Tmax == t_stop
-    while (time < t_stop && n < max_steps) {
-        double dt;
-        int e = rpfvm_get_delta_t(x, &dt);
-        if (e) return e;
-        if (time + dt > t_stop) dt = t_stop - time;
-        if (time + dt >= t_stop) dt = t_stop - time;
-        // The reference loop spins forever when dt stops advancing the clock (a
-        // blown-up state with huge wave speeds); here that ends the loop with the
-        // non-finite-dt fault bit instead.
-        if (!(time + dt > time)) {
-            x->host_pending_faults |= RPFVM_FAULT_NONFINITE_DT;
-            break;
+    if (*t < t_stop && max_steps > 0) {
+        launch_loop_init(x->ctl, *t, t_stop, x->stream);
+        int enq = 0, chunk = 4;
+        // U and U3 trade places after each step the host enqueues; a step the device does
+        // not take leaves them, so the host undoes the swaps of the steps not taken
+        for (;;) {
+            const int k = std::min(chunk, max_steps - enq);
+            for (int q = 0; q < k; q++) {
+                {
+                    Timed tm(x, RPFVM_KCLASS_DT);
+                    launch_dt_loop(x->U, x->g, x->ph, x->slots_valid ? x->slots : DtSlots{},
+                                   x->partial, x->ctl, x->faults, x->stream);
+                }
+                enqueue_stages(x);
+            }
+            enq += k;
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(x->host_ctl, x->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+            const StepCtl c = *x->host_ctl;
+            n = (int)c.steps;
+            if (c.stop || enq >= max_steps) break;
+            // the steps left at the last step's dt, with 5% slack (more rounds if dt shrinks)
+            const double left = c.dt > 0.0 ? (t_stop - c.t) / c.dt : 1.0;
+            chunk = (int)std::min<double>(256.0, std::max<double>(1.0, std::ceil(1.05 * left) + 1.0));
         }
-        if ((e = enqueue_step(x, dt))) return e;
-        time = time + dt;  // == x->time
-        n++;
+        if (x->U3 && ((enq - n) & 1)) std::swap(x->U, x->U3);
+        x->time = x->host_ctl->t;
     }
-    HIPCHK(hipStreamSynchronize(x->stream));
-    *t = time;
+    *t = x->time;
     if (steps) *steps = n;
     return 0;
 }

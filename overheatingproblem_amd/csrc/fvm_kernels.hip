@@ -27,9 +27,12 @@
 namespace fvm {
 
 // --------------------------------------------------------------- ghost fill
-__global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s) {
+__global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s, const StepCtl* __restrict__ step) {
+    if (!step->active) return;
     const int t = s.lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (t > s.hi) return;
+    // the stage time as the driver forms it: t, t + dt, t + 0.5 dt
+    s.t = s.c == 0.0 ? step->t0 : step->t0 + s.c * step->dt;
     bc_side_point(U, g, s, t, blockIdx.y);
 }
 
@@ -40,6 +43,7 @@ __global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s) {
 template <int DIR, int ORDER, bool CHAR, int SOLVER>
 __global__ FLUX_BOUNDS void k_flux(const double* __restrict__ Y, double* __restrict__ F,
                                               Geom g, Phys ph, unsigned* __restrict__ faults) {
+    if (!step_on(ph)) return;
     const int j = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64 + threadIdx.x;
     const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.R - GC || j >= g.C - GC) return;
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(64 * kRingWaves, RPFVM_RING_OCC) void k_flux_ring(c
                                                                  double* __restrict__ F, Geom g,
                                                                  Phys ph, unsigned* __restrict__ faults) {
     __shared__ double lds[kRingWaves * kRingLds];
+    if (!step_on(ph)) return;
     const Tile2 b = xcd_tile();
     const int t = threadIdx.x;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
@@ -205,11 +210,12 @@ struct FaceOp64 {
 template <int ORDER, bool CHAR, int SOLVER>
 __global__ FUSED_BOUNDS void k_yflux_update(const double* Un, const double* Y,
                                                       double* out, const double* __restrict__ F,
-                                                      double* G, Geom g, Phys ph, double dt,
+                                                      double* G, Geom g, Phys ph,
                                                       int kind, Repair rp, DtSlots ds,
                                                       unsigned* __restrict__ faults) {
+    if (!step_on(ph)) return;
     FaceOp64<ORDER, CHAR, SOLVER> op{ph, make_consts(ph), faults};
-    yflux_update_body<false>(Un, Y, out, F, G, g, ph, dt, kind, rp, ds, op);
+    yflux_update_body<false>(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, ds, op);
 }
 
 // The marching form (yflux_update_march) of k_yflux_update: identical results, 8% faster
@@ -220,10 +226,11 @@ __global__ FUSED_BOUNDS void k_yflux_update(const double* Un, const double* Y,
 template <int ORDER, bool CHAR, int SOLVER>
 __global__ __launch_bounds__(64 * kMarchWaves, RPFVM_MARCH_OCC) void k_yflux_march(
     const double* Un, const double* Y, double* out, const double* __restrict__ F, double* G, Geom g,
-    Phys ph, double dt, int kind, Repair rp, DtSlots ds, unsigned* __restrict__ faults) {
+    Phys ph, int kind, Repair rp, DtSlots ds, unsigned* __restrict__ faults) {
     __shared__ double lds[kMarchWaves * kMarchLds];
+    if (!step_on(ph)) return;
     FaceOp64<ORDER, CHAR, SOLVER> op{ph, make_consts(ph), faults};
-    yflux_update_march(Un, Y, out, F, G, g, ph, dt, kind, rp, ds, op, lds);
+    yflux_update_march(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, ds, op, lds);
 }
 
 // ------------------------------------------------------------------ update
@@ -231,11 +238,12 @@ __global__ __launch_bounds__(64 * kMarchWaves, RPFVM_MARCH_OCC) void k_yflux_mar
 __global__ __launch_bounds__(256) void k_update(const double* Un, const double* Y, double* out,
                                                 const double* __restrict__ F,
                                                 const double* __restrict__ G, Geom g, Phys ph,
-                                                double dt, int kind, Repair rp) {
+                                                int kind, Repair rp) {
+    if (!step_on(ph)) return;
     const int j = GC + blockIdx.x * 64 + threadIdx.x;
     const int i = GC + blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.R - GC || j >= g.C - GC) return;
-    const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
+    const bool ok = cell_update(Un, Y, out, F, G, g, ph, ph.step->dt, kind, i, j);
     if (rp.cnt && !ok) {
         const unsigned k = atomicAdd(&rp.cnt[0], 1u);
         rp.list[0][k] = (i - GC) * rp.ny + (j - GC);
@@ -243,7 +251,9 @@ __global__ __launch_bounds__(256) void k_update(const double* Un, const double* 
 }
 
 // Copies the ghost frame of src into dst (all four variables).
-__global__ void k_copy_frame(const double* __restrict__ src, double* __restrict__ dst, Geom g) {
+__global__ void k_copy_frame(const double* __restrict__ src, double* __restrict__ dst, Geom g,
+                             const StepCtl* __restrict__ step) {
+    if (!step->active) return;
     const int nframe = 2 * GC * g.C + 2 * GC * (g.R - 2 * GC);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nframe) return;
@@ -403,16 +413,16 @@ static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& p
 
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_yflux_update(const double* Un, const double* Y, double* out, const double* F,
-                                double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                                double* G, const Geom& g, const Phys& ph, int kind,
                                 const Repair& rp, const FaceFix&, unsigned, const DtSlots& ds,
                                 unsigned* faults, hipStream_t s) {
     if (use_buffer_forms(g, ph)) {
         hipLaunchKernelGGL((k_yflux_march<ORDER, CHAR, SOLVER>), yflux_march_grid(g), yflux_march_block(), 0,
-                           s, Un, Y, out, F, G, g, ph, dt, kind, rp, ds, faults);
+                           s, Un, Y, out, F, G, g, ph, kind, rp, ds, faults);
         return;
     }
     hipLaunchKernelGGL((k_yflux_update<ORDER, CHAR, SOLVER>), yflux_update_grid(g), yflux_update_block(), 0,
-                       s, Un, Y, out, F, G, g, ph, dt, kind, rp, ds, faults);
+                       s, Un, Y, out, F, G, g, ph, kind, rp, ds, faults);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
@@ -478,32 +488,94 @@ void launch_riemann(int dir, int solver, const double* Ls, const double* Rs, dou
     else launch_riemann_d<2>(solver, Ls, Rs, F, g, ph, faults, s);
 }
 
-void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st) {
+void launch_bc_side(double* U, const Geom& g, const BcSide& s, const StepCtl* step, hipStream_t st) {
     const int n = s.hi - s.lo + 1;
     if (n <= 0) return;
     dim3 block(64);
     dim3 grid((n + 63) / 64, 4);
-    hipLaunchKernelGGL(k_bc_side, grid, block, 0, st, U, g, s);
+    hipLaunchKernelGGL(k_bc_side, grid, block, 0, st, U, g, s, step);
 }
 
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
-                   const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                   const double* G, const Geom& g, const Phys& ph, int kind,
                    const Repair& rp, hipStream_t s) {
     const int nj = g.C - 2 * GC, ni = g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
-    hipLaunchKernelGGL(k_update, grid, block, 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp);
+    hipLaunchKernelGGL(k_update, grid, block, 0, s, Un, Y, out, F, G, g, ph, kind, rp);
 }
 
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
-                   const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                   const Geom& g, const Phys& ph, int kind, const Repair& rp,
                    unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
-    launch_repair_t<YFacesStored>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults, s);
+    launch_repair_t<YFacesStored>(Un, Y, out, F, G, g, ph, kind, rp, stamp, ds, faults, s);
 }
 
-void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s) {
+void launch_copy_frame(const double* src, double* dst, const Geom& g, const StepCtl* step, hipStream_t s) {
     const int nframe = 2 * GC * g.C + 2 * GC * (g.R - 2 * GC);
-    hipLaunchKernelGGL(k_copy_frame, dim3((nframe + 255) / 256), dim3(256), 0, s, src, dst, g);
+    hipLaunchKernelGGL(k_copy_frame, dim3((nframe + 255) / 256), dim3(256), 0, s, src, dst, g, step);
+}
+
+// ------------------------------------------------------------ step control
+__global__ void k_step_set(StepCtl* step, double dt, double t0) {
+    step->dt = dt;
+    step->t0 = t0;
+    step->active = 1u;
+}
+
+__global__ void k_loop_init(StepCtl* step, double t, double t_stop) {
+    step->t = t;
+    step->t_stop = t_stop;
+    step->steps = 0u;
+    step->stop = 0u;
+    step->active = 0u;
+}
+
+// The device loop's step decision, on the per-block maxima: the driver loop of
+// src/RP_FVM.c:179-191 (and rpfvm_advance_to's host loop) -- no step once t >= t_stop;
+// dt = CFL*min(dx,dy)/(S_x+S_y) (fault bit 2 when not positive and finite, as
+// k_dt_final), clipped to t_stop; a dt that does not advance the clock ends the loop
+// with fault bit 2; otherwise the step runs from t and the clock becomes t + dt.  Once
+// ended, the loop stays ended.
+__global__ void k_dt_loop(const double* __restrict__ partial, int nblocks, Phys ph,
+                          StepCtl* __restrict__ step, unsigned* __restrict__ faults) {
+    double sx = 0.0, sy = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+        sx = nanmax(sx, partial[2 * k]);
+        sy = nanmax(sy, partial[2 * k + 1]);
+    }
+    sx = wave_max(sx);
+    sy = wave_max(sy);
+    if (threadIdx.x != 0) return;
+    const double t = step->t, ts = step->t_stop;
+    if (step->stop || !(t < ts)) {
+        step->stop = 1u;
+        step->active = 0u;
+        return;
+    }
+    double dt = ph.cfl * min2(ph.dx, ph.dy) / (sx + sy);
+    if (!(dt > 0.0) || !isfinite(dt)) atomicOr(faults, 2u);
+    if (t + dt > ts) dt = ts - t;
+    if (t + dt >= ts) dt = ts - t;
+    if (!(t + dt > t)) {
+        atomicOr(faults, 2u);
+        step->stop = 1u;
+        step->active = 0u;
+        return;
+    }
+    step->dt = dt;
+    step->t0 = t;
+    step->t = t + dt;
+    step->steps += 1u;
+    step->active = 1u;
+}
+
+void launch_step_set(StepCtl* step, double dt, double t0, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(1), 0, s, step, dt, t0);
+}
+
+void launch_loop_init(StepCtl* step, double t, double t_stop, hipStream_t s) {
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, s, step, t, t_stop);
 }
 
 int dt_blocks(const Geom& g) {
@@ -523,6 +595,20 @@ void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlo
     const int nb = (int)std::min<long>(dt_blocks(g), std::max<long>(1, (slots + 255) / 256));
     hipLaunchKernelGGL(k_dt_slots, dim3(nb), dim3(256), 0, s, U, g, ph, ds, partial);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, partial, nb, ph, dt_out, clock, faults);
+}
+
+void launch_dt_loop(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
+                    double* partial, StepCtl* step, unsigned* faults, hipStream_t s) {
+    if (ds.slot) {
+        const long slots = dt_slot_count(g);
+        const int nb = (int)std::min<long>(dt_blocks(g), std::max<long>(1, (slots + 255) / 256));
+        hipLaunchKernelGGL(k_dt_slots, dim3(nb), dim3(256), 0, s, U, g, ph, ds, partial);
+        hipLaunchKernelGGL(k_dt_loop, dim3(1), dim3(64), 0, s, partial, nb, ph, step, faults);
+    } else {
+        const int nb = dt_blocks(g);
+        hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(256), 0, s, U, g, ph, partial);
+        hipLaunchKernelGGL(k_dt_loop, dim3(1), dim3(64), 0, s, partial, nb, ph, step, faults);
+    }
 }
 
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,

@@ -45,6 +45,21 @@ struct Geom {
     size_t plane;   // plane stride in doubles
 };
 
+// The step's control block in device memory.  Every kernel of a step reads dt and the
+// stage clock from it and does nothing when `active` is 0, so a step can be enqueued
+// before its dt is known: rpfvm_time_advance writes a host dt into it (k_step_set), the
+// device-side loop of rpfvm_advance_to has the CFL reduction write it (k_dt_loop).
+struct StepCtl {
+    double dt;        // the step's dt
+    double t0;        // clock at the step's start: stage c runs its ghost fill at t0 + c dt
+    double t;         // device loop: the clock after the last step taken
+    double t_stop;    // device loop: the end time
+    unsigned active;  // 1: the step's kernels run
+    unsigned steps;   // device loop: steps taken
+    unsigned stop;    // device loop: ended (t_stop reached, or dt did not advance the clock)
+    unsigned pad;
+};
+
 // Physical and scheme parameters read by the kernels.
 struct Phys {
     double gamma, dx, dy, cfl, gravity;
@@ -53,7 +68,11 @@ struct Phys {
     int source, positivity;
     int force_fix;   // mixed form: send every face to the FP64 recompute list
     int large_forms;  // the kernel forms for arrays of 4 GB and more, at any size
+    const StepCtl* step;  // the step's control block (null where no step runs)
 };
+
+// Kernel entry of a step: false when the step is not taken (device loop past its end).
+__device__ __forceinline__ bool step_on(const Phys& ph) { return ph.step->active != 0u; }
 
 enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXED = 4,
              BC_DMR_WALL = 5, BC_DMR_SHOCK = 6 };
@@ -62,13 +81,15 @@ enum : int { BC_OUT = 0, BC_REFLECT = 1, BC_PERIODIC = 2, BC_INFLOW = 3, BC_FIXE
 // range [lo, hi] (inclusive); value[] is the inflow or fixed state when used.  The
 // double-Mach-reflection sides also use alt[] (state right of the shock), par[] (wall
 // start, or shock start / height / slope / speed), the cell-centre map x_i =
-// ((i + 0.5 - GC) / nx) * lx and the stage time t.
+// ((i + 0.5 - GC) / nx) * lx and the stage time t (k_bc_side sets it from the step's
+// clock: t0 + c dt).
 struct BcSide {
     int side, type, lo, hi;
     double value[4];
     double alt[4];
     double par[4];
     double nx, lx, t;
+    double c;  // the stage's time coefficient (SSP-RK3: 0, 1, 1/2)
 };
 
 enum : int { UPD_RESIDUAL = 0, UPD_EULER = 1, UPD_RK2_2 = 2, UPD_RK3_2 = 3, UPD_RK3_3 = 4 };
@@ -113,18 +134,18 @@ using FluxLaunch = void (*)(const double* Y, double* F, const Geom& g, const Phy
 // y faces fused with the residual + stage combine (out must not alias Y); `stamp` is
 // the stage's unique stamp for de-duplicating safeguard list entries.
 using FusedLaunch = void (*)(const double* Un, const double* Y, double* out, const double* F,
-                             double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                             double* G, const Geom& g, const Phys& ph, int kind,
                              const Repair& rp, const FaceFix& fix, unsigned stamp,
                              const DtSlots& ds, unsigned* faults, hipStream_t s);
 // The positivity safeguard's rounds after a stage (fvm_repair.hpp).
 using RepairLaunch = void (*)(const double* Un, const double* Y, double* out, double* F, double* G,
-                              const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                              const Geom& g, const Phys& ph, int kind, const Repair& rp,
                               unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
 
 // The whole stage in one pass (fvm_onepass.hpp): x faces, y faces, residual and stage
 // combine (out must not alias Y; buffer forms only).
 using StageLaunch = void (*)(const double* Un, const double* Y, double* out, double* F, double* G,
-                             const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                             const Geom& g, const Phys& ph, int kind, const Repair& rp,
                              const FaceFix& fix, unsigned stamp, const DtSlots& ds,
                              unsigned* faults, hipStream_t s);
 // After a stage of the mixed form: the FP64 recomputation of the deferred faces, the update
@@ -146,16 +167,24 @@ FluxPair select_flux_mixed(int order, bool characteristic, int solver);
 // FP64 solver `solver` on face states Ls/Rs into F, faces of direction dir (1 or 2).
 void launch_riemann(int dir, int solver, const double* Ls, const double* Rs, double* F,
                     const Geom& g, const Phys& ph, unsigned* faults, hipStream_t s);
-void launch_bc_side(double* U, const Geom& g, const BcSide& s, hipStream_t st);
+void launch_bc_side(double* U, const Geom& g, const BcSide& s, const StepCtl* step, hipStream_t st);
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
-                   const double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                   const double* G, const Geom& g, const Phys& ph, int kind,
                    const Repair& rp, hipStream_t s);
 void launch_repair(const double* Un, const double* Y, double* out, double* F, double* G,
-                   const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                   const Geom& g, const Phys& ph, int kind, const Repair& rp,
                    unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s);
 // Clears the mixed form's FP64 recompute lists (after the space operator's sweeps).
 void launch_fix_reset(const FaceFix& fix, hipStream_t s);
-void launch_copy_frame(const double* src, double* dst, const Geom& g, hipStream_t s);
+void launch_copy_frame(const double* src, double* dst, const Geom& g, const StepCtl* step, hipStream_t s);
+// Writes a host dt and clock into the step's control block and marks the step active.
+void launch_step_set(StepCtl* step, double dt, double t0, hipStream_t s);
+// Starts the device loop: clock t, end t_stop, no steps taken.
+void launch_loop_init(StepCtl* step, double t, double t_stop, hipStream_t s);
+// The device loop's CFL reduction: dt from the slots (ds.slot) or a full pass, clipped to
+// the loop's end as the driver does (src/RP_FVM.c:186-191); sets the next step's control.
+void launch_dt_loop(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
+                    double* partial, StepCtl* step, unsigned* faults, hipStream_t s);
 int dt_blocks(const Geom& g);
 void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, double* dt_out,
                const double* clock, unsigned* faults, hipStream_t s);

@@ -61,6 +61,7 @@ template <int DIR, int ORDER, bool CHAR, int SOLVER, bool BUF>
 __global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
                                                     double* __restrict__ F, Geom g, Phys ph,
                                                     FaceFix fix) {
+    if (!step_on(ph)) return;
     const int j0 = (DIR == 1 ? GC : GC - 1) + blockIdx.x * 64;
     const int j = j0 + threadIdx.x;
     const int i = (DIR == 1 ? GC - 1 : GC) + blockIdx.y * blockDim.y +
@@ -95,6 +96,7 @@ template <int DIR, int ORDER, bool CHAR, int SOLVER>
 __global__ __launch_bounds__(256) void k_flux_fix(const double* __restrict__ Y,
                                                   double* __restrict__ F, Geom g, Phys ph,
                                                   FaceFix fix, unsigned* __restrict__ faults) {
+    if (!step_on(ph)) return;
     const unsigned n = min(fix.cnt[DIR - 1], fix.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         fix.cnt[2 - DIR] = 0;
@@ -159,11 +161,12 @@ template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
 __global__ FUSED_MX_BOUNDS void k_yflux_update_mixed(const double* Un, const double* Y,
                                                             double* out,
                                                             const double* __restrict__ F,
-                                                            double* G, Geom g, Phys ph, double dt,
+                                                            double* G, Geom g, Phys ph,
                                                             int kind, Repair rp, FaceFix fix,
                                                             DtSlots ds) {
+    if (!step_on(ph)) return;
     FaceOpMx<ORDER, CHAR, SOLVER> op{ph, fix};
-    yflux_update_body<true>(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, op);
+    yflux_update_body<true>(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, SLOTS ? ds : DtSlots{}, op);
 }
 
 // The marching form of the same kernel (yflux_update_march): identical results.  Used
@@ -177,33 +180,34 @@ template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
 __global__ __launch_bounds__(64 * kMarchWaves, 4) void k_yflux_march_mixed(const double* Un, const double* Y,
                                                            double* out,
                                                            const double* __restrict__ F,
-                                                           double* G, Geom g, Phys ph, double dt,
+                                                           double* G, Geom g, Phys ph,
                                                            int kind, Repair rp, FaceFix fix,
                                                            DtSlots ds) {
     __shared__ double lds[kMarchWaves * kMarchLds];
+    if (!step_on(ph)) return;
     FaceOpMx<ORDER, CHAR, SOLVER> op{ph, fix};
-    yflux_update_march(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, op, lds);
+    yflux_update_march(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, SLOTS ? ds : DtSlots{}, op, lds);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_yflux_update_mixed(const double* Un, const double* Y, double* out,
                                       const double* F, double* G, const Geom& g, const Phys& ph,
-                                      double dt, int kind, const Repair& rp, const FaceFix& fix,
+                                      int kind, const Repair& rp, const FaceFix& fix,
                                       unsigned stamp, const DtSlots& ds, unsigned* faults,
                                       hipStream_t s) {
     if (use_buffer_forms(g, ph)) {
         if (ds.slot)
             hipLaunchKernelGGL((k_yflux_march_mixed<ORDER, CHAR, SOLVER, true>), yflux_march_grid(g),
-                               yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                               yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
         else
             hipLaunchKernelGGL((k_yflux_march_mixed<ORDER, CHAR, SOLVER, false>), yflux_march_grid(g),
-                               yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                               yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
     } else if (ds.slot) {
         hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER, true>), yflux_update_grid(g),
-                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
     } else {
         hipLaunchKernelGGL((k_yflux_update_mixed<ORDER, CHAR, SOLVER, false>), yflux_update_grid(g),
-                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                           yflux_update_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
     }
 }
 
@@ -273,25 +277,26 @@ struct YFacesMx {
 template <int ORDER, bool CHAR, int SOLVER, bool SLOTS>
 __global__ __launch_bounds__(64 * kOpWaves, (op_occupancy<ORDER, CHAR>())) void k_stage_mixed(const double* Un, const double* Y,
                                                                   double* out, double* F, double* G,
-                                                                  Geom g, Phys ph, double dt, int kind,
+                                                                  Geom g, Phys ph, int kind,
                                                                   Repair rp, FaceFix fix, DtSlots ds) {
     __shared__ double lds[kOpWaves * kOpLds];
+    if (!step_on(ph)) return;
     FaceOpMxX<ORDER, CHAR, SOLVER> opx{ph, fix};
     FaceOpMx<ORDER, CHAR, SOLVER> opy{ph, fix};
-    stage_march(Un, Y, out, F, G, g, ph, dt, kind, rp, SLOTS ? ds : DtSlots{}, opx, opy, lds);
+    stage_march(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, SLOTS ? ds : DtSlots{}, opx, opy, lds);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_stage_mixed(const double* Un, const double* Y, double* out, double* F,
-                               double* G, const Geom& g, const Phys& ph, double dt, int kind,
+                               double* G, const Geom& g, const Phys& ph, int kind,
                                const Repair& rp, const FaceFix& fix, unsigned stamp,
                                const DtSlots& ds, unsigned* faults, hipStream_t s) {
     if (ds.slot)
         hipLaunchKernelGGL((k_stage_mixed<ORDER, CHAR, SOLVER, true>), stage_march_grid(g),
-                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
     else
         hipLaunchKernelGGL((k_stage_mixed<ORDER, CHAR, SOLVER, false>), stage_march_grid(g),
-                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, dt, kind, rp, fix, ds);
+                           stage_march_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
 }
 
 // The x-face fluxes of the safeguard's re-updated cells after a one-pass stage, which stores
@@ -334,6 +339,7 @@ template <int ORDER, bool CHAR, int SOLVER>
 __global__ __launch_bounds__(256) void k_fixfaces_mx(const double* __restrict__ Y, double* F, double* G,
                                                      Geom g, Phys ph, FaceFix fix,
                                                      unsigned* __restrict__ faults) {
+    if (!step_on(ph)) return;
     const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n0 | n1)) atomicOr(faults, 4u);
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n0 + n1; q += gridDim.x * blockDim.x) {
@@ -357,8 +363,10 @@ __global__ __launch_bounds__(256) void k_fixfaces_mx(const double* __restrict__ 
 __global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const double* Y, double* out,
                                                       const double* __restrict__ F,
                                                       const double* __restrict__ G, Geom g, Phys ph,
-                                                      double dt, int kind, Repair rp, FaceFix fix,
+                                                      int kind, Repair rp, FaceFix fix,
                                                       unsigned stamp, DtSlots ds) {
+    if (!step_on(ph)) return;
+    const double dt = ph.step->dt;
     const unsigned n0 = min(fix.cnt[0], fix.cap), n1 = min(fix.cnt[1], fix.cap);
     const unsigned tstamp = stamp * 16u + 15u;
     for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (n0 + n1); q += gridDim.x * blockDim.x) {
@@ -401,14 +409,14 @@ __global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const do
 
 template <int ORDER, bool CHAR, int SOLVER, typename YF, typename XF>
 static void launch_fixup(const double* Un, const double* Y, double* out, double* F, double* G,
-                         const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                         const Geom& g, const Phys& ph, int kind, const Repair& rp,
                          const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
                          hipStream_t s) {
     hipLaunchKernelGGL((k_fixfaces_mx<ORDER, CHAR, SOLVER>), dim3(kFixFaceBlocks), dim3(256), 0, s, Y, F,
                        G, g, ph, fix, faults);
-    hipLaunchKernelGGL(k_fixcells_mx, dim3(kFixCellBlocks), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, dt,
+    hipLaunchKernelGGL(k_fixcells_mx, dim3(kFixCellBlocks), dim3(1024), 0, s, Un, Y, out, F, G, g, ph,
                        kind, rp, fix, stamp, ds);
-    if (rp.cnt) launch_repair_t<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults, s);
+    if (rp.cnt) launch_repair_t<YF, XF>(Un, Y, out, F, G, g, ph, kind, rp, stamp, ds, faults, s);
 }
 
 // After the two-sweep stage: every x face is in F (the fix-up writes the deferred ones);
@@ -416,25 +424,25 @@ static void launch_fixup(const double* Un, const double* Y, double* out, double*
 // 4 GB and more) every one.
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_fixup2(const double* Un, const double* Y, double* out, double* F, double* G,
-                          const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                          const Geom& g, const Phys& ph, int kind, const Repair& rp,
                           const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
                           hipStream_t s) {
     if (use_buffer_forms(g, ph))
         launch_fixup<ORDER, CHAR, SOLVER, YFacesMx<ORDER, CHAR, SOLVER>, XFacesStored>(
-            Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, faults, s);
+            Un, Y, out, F, G, g, ph, kind, rp, fix, stamp, ds, faults, s);
     else
-        launch_fixup<ORDER, CHAR, SOLVER, YFacesStored, XFacesStored>(Un, Y, out, F, G, g, ph, dt, kind,
+        launch_fixup<ORDER, CHAR, SOLVER, YFacesStored, XFacesStored>(Un, Y, out, F, G, g, ph, kind,
                                                                       rp, fix, stamp, ds, faults, s);
 }
 
 // After the one-pass stage, which stores only the faces of either direction the fix-up reads.
 template <int ORDER, bool CHAR, int SOLVER>
 static void launch_fixup1(const double* Un, const double* Y, double* out, double* F, double* G,
-                          const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                          const Geom& g, const Phys& ph, int kind, const Repair& rp,
                           const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
                           hipStream_t s) {
     launch_fixup<ORDER, CHAR, SOLVER, YFacesMx<ORDER, CHAR, SOLVER>, XFacesMx<ORDER, CHAR, SOLVER>>(
-        Un, Y, out, F, G, g, ph, dt, kind, rp, fix, stamp, ds, faults, s);
+        Un, Y, out, F, G, g, ph, kind, rp, fix, stamp, ds, faults, s);
 }
 
 template <int ORDER, bool CHAR, int SOLVER>

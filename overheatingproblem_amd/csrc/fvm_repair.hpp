@@ -174,17 +174,18 @@ __device__ __forceinline__ void repair_rounds(const double* Un, const double* Y,
 template <typename YF, typename XF>
 __global__ __launch_bounds__(1024) void k_repair_block(const double* Un, const double* Y,
                                                        double* out, double* F, double* G, Geom g,
-                                                       Phys ph, double dt, int kind, Repair rp,
+                                                       Phys ph, int kind, Repair rp,
                                                        unsigned stamp, DtSlots ds,
                                                        unsigned* __restrict__ faults) {
-    repair_rounds<YF, XF>(Un, Y, out, F, G, g, ph, dt, kind, rp, stamp, ds, faults);
+    if (!step_on(ph)) return;
+    repair_rounds<YF, XF>(Un, Y, out, F, G, g, ph, ph.step->dt, kind, rp, stamp, ds, faults);
 }
 
 template <typename YF, typename XF = XFacesStored>
 void launch_repair_t(const double* Un, const double* Y, double* out, double* F, double* G,
-                     const Geom& g, const Phys& ph, double dt, int kind, const Repair& rp,
+                     const Geom& g, const Phys& ph, int kind, const Repair& rp,
                      unsigned stamp, const DtSlots& ds, unsigned* faults, hipStream_t s) {
-    hipLaunchKernelGGL((k_repair_block<YF, XF>), dim3(1), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, dt, kind,
+    hipLaunchKernelGGL((k_repair_block<YF, XF>), dim3(1), dim3(1024), 0, s, Un, Y, out, F, G, g, ph, kind,
                        rp, stamp, ds, faults);
 }
 

@@ -219,6 +219,18 @@ def accuracy_markdown(d: dict, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def accuracy_summary(d: dict) -> dict:
+    """The figures of the Noh full-length record that bench.py's line carries."""
+    fr = d["final_regions"]
+    return {"t_end": d["t_end"], "steps_fp64": d["fp64"]["steps"],
+            "l1_rho_mixed_own_dts": fr["mixed"]["l1"],
+            "l1_rho_mixed_fp64_dts": fr["mixed_replay"]["l1"],
+            "l1_rho_fp64_perturbed_1e-12": fr["fp64_perturbed"]["l1"],
+            "l1_rho_rest_mixed_fp64_dts": fr["mixed_replay"]["l1_rest"],
+            "l1_rho_rest_fp64_perturbed_1e-12": fr["fp64_perturbed"]["l1_rest"],
+            "masked": d["regions"]}
+
+
 def dmr_markdown(d: dict, command: str) -> str:
     """The double-Mach-reflection full-length record from `bench.py --dmr-full`."""
     regions = ("fronts", "rest")
@@ -364,6 +376,7 @@ def main(argv):
         path = os.path.join(PROFILES, f"{name}_accuracy.md")
         with open(path, "w") as fh:
             fh.write(accuracy_markdown(d, command))
+        _merge(d["workload"].replace("-", "").split("x")[0], "accuracy_full", accuracy_summary(d))
         print(path)
         return 0
     if len(argv) >= 4 and argv[0] in ("dmr", "c1"):

@@ -1,0 +1,120 @@
+"""The device-side step loop of rpfvm_advance_to (dt, clipping and clock on the device,
+steps enqueued in chunks, kernels of steps past the end doing nothing) against the
+host-driven loop of the same driver semantics (src/RP_FVM.c:179-191: dt from
+rpfvm_get_delta_t, clipped to t_stop on the host, then rpfvm_time_advance): the same
+states, times and step counts, bit for bit."""
+import numpy as np
+import pytest
+
+from overheatingproblem_amd import cases
+from overheatingproblem_amd.capi import Solver
+from overheatingproblem_amd.config import FAULT_NONFINITE_DT, PRECISION_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+def host_loop(s, t, t_stop, max_steps):
+    """src/RP_FVM.c:179-191 with next_output_time == Tmax == t_stop."""
+    s.set_time(t)
+    n = 0
+    while t < t_stop and n < max_steps:
+        dt = s.get_delta_t()
+        if t + dt > t_stop:
+            dt = t_stop - t
+        if t + dt >= t_stop:
+            dt = t_stop - t
+        if not (t + dt > t):
+            break
+        s.time_advance(dt)
+        t = t + dt
+        n += 1
+    return t, n
+
+
+def both(cfg, U0, t_stop, max_steps, t0=0.0):
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        th, nh = host_loop(s, t0, t_stop, max_steps)
+        Uh = s.get_state()
+        fh = s.sync()
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        td, nd = s.advance_to(t_stop, max_steps, t0)
+        Ud = s.get_state()
+        fd = s.sync()
+        assert s.get_time() == td
+    return (Uh, th, nh, fh), (Ud, td, nd, fd)
+
+
+def check(cfg, U0, t_stop, max_steps, t0=0.0):
+    (Uh, th, nh, fh), (Ud, td, nd, fd) = both(cfg, U0, t_stop, max_steps, t0)
+    assert nd == nh and td == th and fd == fh
+    assert np.array_equal(Ud, Uh), f"{int((Ud != Uh).sum())} entries differ"
+    return nd, td
+
+
+def mixed(cfg):
+    c = cfg.copy()
+    c.precision = PRECISION_MIXED
+    return c
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_to_the_end_time(form):
+    """Stops exactly at t_stop (the last dt clipped) well inside a chunk."""
+    c, U0 = cases.lax_liu_3(96)
+    n, t = check(c if form == "fp64" else mixed(c), U0, 0.02, 10 ** 6)
+    assert t == 0.02 and n > 4
+
+
+@pytest.mark.parametrize("steps", [1, 3, 4, 5, 37])
+def test_max_steps(steps):
+    c, U0 = cases.lax_liu_3(64)
+    n, _ = check(c, U0, 1.0, steps)
+    assert n == steps
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_safeguard_state_buffers(form):
+    """With the safeguard the state trades buffers each step: an odd and an even number
+    of steps taken, and steps enqueued past the end that the device does not take."""
+    c, U0 = cases.noh(96, positivity=True)
+    c = c if form == "fp64" else mixed(c)
+    for t_stop in (0.004, 0.0045, 0.02):
+        n, _ = check(c, U0, t_stop, 10 ** 6)
+        assert n > 0
+
+
+def test_time_dependent_sides():
+    """The double-Mach-reflection sides read the stage clock t0 + c dt from the step's
+    control block."""
+    c, U0 = cases.dmr(96, 24)
+    check(c, U0, 0.01, 10 ** 6)
+
+
+def test_rk_orders_and_start_time():
+    for order in (1, 2):
+        c, U0 = cases.lax_liu_3(64, time_order=order)
+        check(c, U0, 0.03, 10 ** 6, t0=0.01)
+
+
+def test_nothing_to_do():
+    c, U0 = cases.lax_liu_3(32)
+    with Solver(c) as s:
+        s.set_state(U0)
+        assert s.advance_to(0.5, 100, 0.5) == (0.5, 0)
+        assert s.advance_to(0.5, 0, 0.0) == (0.0, 0)
+        assert np.array_equal(s.get_state(), U0)
+
+
+def test_non_finite_dt_ends_the_loop():
+    """A state whose wave speeds are not finite gives a dt that does not advance the clock:
+    the device loop ends there with the fault bit, as the host loop does."""
+    c, U0 = cases.lax_liu_3(32)
+    U = U0.copy()
+    U[3, 10:12, 10:12] = np.inf  # infinite energy: infinite sound speed
+    with Solver(c) as s:
+        s.set_state(U)
+        t, n = s.advance_to(0.1, 100)
+        assert n == 0 and t == 0.0
+        assert s.sync() & FAULT_NONFINITE_DT
