@@ -58,6 +58,25 @@ def full_length_record(n):
     return dict(rec, source=f"profiles/noh{n}_t0.6_accuracy.md (python bench.py --noh-full)")
 
 
+def one_pass_record(prof, rop, rmx, cells, steps, alg_bytes, value):
+    """The mixed form's one-pass stage (x faces, y faces and update in one kernel, forced
+    with RPFVM_DEBUG_ONE_PASS) beside the default two-sweep stage: throughput and, from
+    the committed rocprofv3 counters, the HBM bytes one stage moves."""
+    v = cells * steps / (rop["gpu_ms"] / 1e3) / 1e6
+    k = prof.get("kernels", {})
+    rec = {"value": round(v, 3), "ms_per_step": round(rop["gpu_ms"] / steps, 4),
+           "vs_two_sweep": round(v / value, 4), "faults": rop["faults"],
+           "same_state_as_two_sweep": bool(np.array_equal(rop["U"], rmx["U"])),
+           "kernel": "k_stage_mixed"}
+    if "k_stage_mixed" in k:
+        tb = k["k_stage_mixed"]["traffic_bytes"]
+        rec.update({"traffic": round(tb), "traffic_vs_alg": round(tb / alg_bytes, 3),
+                    "two_sweep_stage_traffic_vs_alg": round(
+                        (k["k_flux_mixed"]["traffic_bytes"] + k["k_yflux_march_mixed"]["traffic_bytes"])
+                        / alg_bytes, 3) if "k_flux_mixed" in k and "k_yflux_march_mixed" in k else None})
+    return rec
+
+
 def kernel_io_bytes(cls, nx, ny, stores_g=True):
     """Bytes one launch of a timing class must read and write (FP64, 4 vars), averaged
     over the three RK3 stages.  x sweep: the stage state and the x-face fluxes.  Fused
@@ -716,7 +735,7 @@ def main():
         print(json.dumps(ll3_full()), flush=True)
         return
     from overheatingproblem_amd import cases
-    from overheatingproblem_amd.config import PRECISION_MIXED
+    from overheatingproblem_amd.config import DEBUG_ONE_PASS, PRECISION_MIXED
     n = args.n
     cfg, U0 = cases.noh(n, positivity=True)
     cfg_mx = cfg.copy()
@@ -732,7 +751,11 @@ def main():
     rkt = run_gpu(cfg_mx, U0, args.steps, args.warmup)
     rmx["kernels"] = rkt["kernels"]
     log(f"bench: mixed {rmx['gpu_ms'] / args.steps:.3f} ms/step ({rkt['gpu_ms'] / args.steps:.3f} "
-        f"with per-kernel events); host-driven fp64 dts ...")
+        f"with per-kernel events); mixed one-pass stage ...")
+    cfg_op = cfg_mx.copy()
+    cfg_op.debug_flags = DEBUG_ONE_PASS
+    rop = run_gpu(cfg_op, U0, args.steps, args.warmup, timing=False)
+    log(f"bench: mixed one-pass {rop['gpu_ms'] / args.steps:.3f} ms/step; host-driven fp64 dts ...")
     dts64, U64h = host_loop_dts(cfg, U0, args.warmup + args.steps)
     log("bench: mixed replay on the fp64 dts (host-driven loop) ...")
     rep = run_gpu(cfg_mx, U0, args.steps, args.warmup, dts=dts64, timing=False)
@@ -831,6 +854,7 @@ def main():
                      "l1_p_mixed_vs_fp64": l1p, "linf_p_mixed_vs_fp64": linfp,
                      "note": f"mixed replayed on the fp64 run's {len(dts64)} dts",
                      "full_length": full_length_record(n)},
+        "one_pass_stage": one_pass_record(prof, rop, rmx, cells, args.steps, alg_bytes, value),
         "kernel_timing": {"source": "a second timed run of the same steps with the library's "
                                     "per-kernel HIP events on the context stream",
                           "mixed_ms_per_step_with_events": round(rkt["gpu_ms"] / args.steps, 4)},
