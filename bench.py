@@ -174,11 +174,12 @@ def run_gpu(cfg, U0, steps, warmup, dts=None, timing=True):
     ev = HipEvents()
     with Solver(cfg) as s:
         s.set_state(U0)
+        tw = 0.0
         if dts is not None:
             for dt in dts[:warmup]:
                 s.time_advance(dt)
         else:
-            s.advance_to(float("inf"), warmup)
+            tw, _ = s.advance_to(float("inf"), warmup)
         faults = s.sync()
         stream = s.stream()
         s.set_kernel_timing(timing)
@@ -190,8 +191,8 @@ def run_gpu(cfg, U0, steps, warmup, dts=None, timing=True):
             for dt in dts[warmup:warmup + steps]:
                 s.time_advance(dt)
             n = len(dts[warmup:warmup + steps])
-        else:
-            _, n = s.advance_to(float("inf"), steps)
+        else:  # from the warm-up's end: the moving boundary sides read the clock
+            _, n = s.advance_to(float("inf"), steps, tw)
         ev.record(e1, stream)
         gpu_ms = ev.elapsed_ms(e0, e1)
         t1 = time.perf_counter()
@@ -287,12 +288,30 @@ def c1_full(cores):
         U = s.get_state()
         faults = s.sync()
     cells = cfg.nx * cfg.ny
+    log("bench: GPU FP64 through the device-side loop ...")
+    with Solver(cfg) as s:
+        s.set_state(U0)
+        stream = s.stream()
+        e0, e1 = ev.create(), ev.create()
+        t0 = time.perf_counter()
+        ev.record(e0, stream)
+        td, nd = s.advance_to(t_end, 10 ** 7)
+        ev.record(e1, stream)
+        gpu_ms_d = ev.elapsed_ms(e0, e1)
+        wall_d = time.perf_counter() - t0
+        Ud = s.get_state()
+        faults_d = s.sync()
 
     def same(a, b):
         a, b = np.asarray(a), np.asarray(b)
         return bool(a.shape == b.shape and np.array_equal(np.where(a == 0, 0.0, a),
                                                           np.where(b == 0, 0.0, b)))
     return {"workload": "sedov-200x200 to t=1.0", "steps": len(rdts),
+            "gpu_fp64_device_loop": {"value": round(cells * nd / wall_d / 1e6, 3),
+                                     "unit": "Mcell-updates/s", "seconds_wall": round(wall_d, 3),
+                                     "gpu_ms": round(gpu_ms_d, 2), "steps": nd, "faults": faults_d,
+                                     "t": td, "state_bit_identical": same(Ud, rU),
+                                     "note": "rpfvm_advance_to: dt, clipping and clock on the device"},
             "cpu_reference": {"value": round(cells * len(rdts) / rec["seconds"] / 1e6, 4),
                               "unit": "Mcell-updates/s", "seconds": round(rec["seconds"], 3),
                               "cores": rec["threads"], "kind": "reference",
@@ -355,7 +374,7 @@ def run_dmr(steps, warmup):
     cells = cfg.nx * cfg.ny
     r64 = run_gpu(cfg, U0, steps, warmup, timing=False)
     rmx = run_gpu(mx, U0, steps, warmup, timing=False)
-    dts, _ = host_loop_dts(cfg, U0, warmup + steps)
+    dts, U64h = host_loop_dts(cfg, U0, warmup + steps)
     rep = run_gpu(mx, U0, steps, warmup, dts=dts, timing=False)
     p64 = cases.cons_to_prim(r64["U"][:, 4:-4, 4:-4], cfg.gamma)
     pmx = cases.cons_to_prim(rep["U"][:, 4:-4, 4:-4], cfg.gamma)
@@ -367,7 +386,8 @@ def run_dmr(steps, warmup):
             "fp64": rate(r64), "mixed": rate(rmx), "speedup": round(rate(rmx) / rate(r64), 4),
             "l1_rho": float(np.abs(pmx[0] - p64[0]).mean()), "linf_rho": float(np.abs(pmx[0] - p64[0]).max()),
             "l1_p": float(np.abs(pmx[3] - p64[3]).mean()), "linf_p": float(np.abs(pmx[3] - p64[3]).max()),
-            "faults": {"fp64": r64["faults"], "mixed": rmx["faults"]}}
+            "faults": {"fp64": r64["faults"], "mixed": rmx["faults"]},
+            "fp64_device_loop_matches_host_driven_loop": bool(np.array_equal(r64["U"], U64h))}
 
 
 def noh_metrics(states, n, gamma, t_end=0.6):

@@ -273,15 +273,21 @@ def c1_markdown(d: dict, command: str) -> str:
         "r <= 2 dx; WENO5-char + HLLC, SSP-RK3, CFL 0.5, outflow, t_end = 1.0.  The reference's "
         "own step (oracle/_ref/ref_harness, mode run: the driver loop of src/RP_FVM.c:179-191, "
         "dt + step timed as src/RP_FVM.c:181-216, IO excluded) on the GPU host's cores, then the "
-        "FP64 form on the GPU through the same loop (one dt read back per step).", "",
+        "FP64 form on the GPU through the same loop (one dt read back per step) and through the "
+        "device-side loop (rpfvm_advance_to: dt, clipping and clock on the device).", "",
         "| run | steps | seconds | Mcell-updates/s | cores |", "|---|---:|---:|---:|---:|",
         f"| reference CPU/OpenMP step | {d['steps']} | {c['seconds']:.2f} | {c['value']:.3f} | "
         f"{c['cores']} ({h['cpu_model']}, {h['cpus_visible']} CPUs visible, OMP_NUM_THREADS="
         f"{h['omp_num_threads_env']}) |",
-        f"| FP64 form, one MI355X | {g['steps']} | {g['seconds_wall']:.2f} | {g['value']:.1f} | |", "",
+        f"| FP64 form, one MI355X, host-driven loop | {g['steps']} | {g['seconds_wall']:.2f} | "
+        f"{g['value']:.1f} | |"] + ([
+        f"| FP64 form, one MI355X, device-side loop | {dl['steps']} | {dl['seconds_wall']:.2f} | "
+        f"{dl['value']:.1f} | |"] if (dl := d.get("gpu_fp64_device_loop")) else []) + ["",
         f"Bit-identical to the reference: dts {b['dts']}, final state {b['state']}, final time "
-        f"{b['t']}.  At 200^2 the GPU loop is bound by one launch sequence and one host "
-        "round trip per step, not by the step's arithmetic."]) + "\n"
+        f"{b['t']}" + (f"; device-side loop: final state {dl['state_bit_identical']}, final "
+                       f"time {dl['t']}." if dl else ".") + "  At 200^2 the step's arithmetic "
+        "is a small part of a step: the host-driven loop pays one launch sequence and one host "
+        "round trip per step, the device-side loop the launch sequence only."]) + "\n"
 
 
 def ll3_markdown(d: dict, command: str) -> str:
