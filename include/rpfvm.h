@@ -53,8 +53,7 @@ enum {
 /* Riemann solver ids as the "scheme" control key, include/CFD_convection.h:89-117.
  * Any other id except 5 selects LLF, as the reference's default branch.  The
  * heat-capacity solvers 11, 22, 33 (HLLHC_Flux, HLLCHC_Flux, RoeHC_Flux,
- * include/scheme.h:1860, 2153, 2793) run in both forms of a context and in FP64
- * batches (not in mixed batches). */
+ * include/scheme.h:1860, 2153, 2793) run in both forms, in a context and in batches. */
 enum {
     RPFVM_SCHEME_HLL = 1, RPFVM_SCHEME_HLLC = 2, RPFVM_SCHEME_ROE = 3, RPFVM_SCHEME_LLF = 10,
     RPFVM_SCHEME_HLLHC = 11, RPFVM_SCHEME_HLLCHC = 22, RPFVM_SCHEME_ROEHC = 33
@@ -261,10 +260,9 @@ int rpfvm_kernel_times(rpfvm_ctx *ctx, double ms[RPFVM_KCLASS_COUNT], int counts
  * Batching form: many independent problems of uneven size advanced together
  * (no counterpart in the reference, whose driver runs one problem per process,
  * src/RP_FVM.c:92-372).  Each problem keeps its own configuration (grid, dx/dy,
- * gamma, CFL, Riemann solver -- the heat-capacity ones in FP64 batches --, boundary
- * conditions, source) and its own dt and clock; recon_order (1, 2, 3 or 5),
- * characteristic, time_order and precision must be the same for every problem, and the
- * positivity safeguard is not available.  One RK
+ * gamma, CFL, Riemann solver, boundary conditions, source) and its own dt and clock;
+ * recon_order (1, 2, 3 or 5), characteristic, time_order, precision and the positivity
+ * safeguard (two-sweep schedule only) must be the same for every problem.  One RK
  * stage of all active problems is one persistent launch driven by a work queue
  * of strips; the whole step is captured once in a HIP graph and replayed.  Each
  * problem's interior evolves bit-identically to an rpfvm_ctx with the same

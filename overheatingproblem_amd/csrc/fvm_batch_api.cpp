@@ -184,13 +184,6 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         for (int s = 0; s < 4; s++)
             if (c.bc[s] > RPFVM_BC_FIXED_VALUE)
                 return fail(RPFVM_ERR_UNSUPPORTED, "problem %d: time-dependent sides are not batched", k);
-        // the mixed batch kernels dispatch HLL / HLLC / Roe / LLF at run time; the
-        // heat-capacity solvers run in the mixed form of rpfvm_ctx and in FP64 batches
-        if (c.precision == RPFVM_PRECISION_MIXED &&
-            (c.scheme == RPFVM_SCHEME_HLLHC || c.scheme == RPFVM_SCHEME_HLLCHC ||
-             c.scheme == RPFVM_SCHEME_ROEHC))
-            return fail(RPFVM_ERR_UNSUPPORTED,
-                        "problem %d: the heat-capacity schemes 11, 22, 33 run in FP64 batches only", k);
         if (c.nx < 2 * GC || c.ny < 2 * GC || c.nx >= (1 << 20) - 2 * GC || c.ny >= (1 << 20) - 2 * GC)
             return fail(RPFVM_ERR_INVALID, "problem %d: nx, ny must be in 8 .. 2^20 - 9", k);
         if ((c.characteristic != 0) != (cfgs[0].characteristic != 0) ||

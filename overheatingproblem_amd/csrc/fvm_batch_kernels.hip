@@ -64,8 +64,8 @@ __device__ __forceinline__ double bload(const double* P, const BProb& d, int i, 
 }
 
 // FP64 face (the single-problem k_flux arithmetic) with the solver chosen at run time.
-// HC adds the heat-capacity solvers (FP64 batches only: the mixed form does not run them,
-// so its FP64 fix-ups leave them out).
+// HC adds the heat-capacity solvers (every FP64 face of a batch but the mixed form's own
+// sweep kernels, which compute FP64 faces through their fix-ups only).
 template <int DIR, int ORDER, bool CHAR, bool HC = false>
 __device__ __forceinline__ int face64(const Consts& kc, const Phys& ph, int solver,
                                       const double st[6][4], double f[4]) {
@@ -95,8 +95,9 @@ __device__ __forceinline__ int face64(const Consts& kc, const Phys& ph, int solv
     }
 }
 
-// Mixed-form face (k_flux_mixed arithmetic); false when the FP32-reconstructed states
-// are not admissible and the face must be recomputed in FP64.
+// Mixed-form face (k_flux_mixed arithmetic, every solver incl. the heat-capacity ones);
+// false when the FP32-reconstructed states are not admissible and the face must be
+// recomputed in FP64.
 template <int DIR, int ORDER, bool CHAR>
 __device__ __forceinline__ bool face_mx(const Phys& ph, int solver, const double st[6][4],
                                         double f[4]) {
@@ -108,6 +109,9 @@ __device__ __forceinline__ bool face_mx(const Phys& ph, int solver, const double
         case SOLVER_HLL: mx::riemann<DIR, SOLVER_HLL>(ph.gamma, UL, UR, f); break;
         case SOLVER_HLLC: mx::riemann<DIR, SOLVER_HLLC>(ph.gamma, UL, UR, f); break;
         case SOLVER_ROE: mx::riemann<DIR, SOLVER_ROE>(ph.gamma, UL, UR, f); break;
+        case SOLVER_HLLHC: mx::riemann<DIR, SOLVER_HLLHC>(ph.gamma, UL, UR, f); break;
+        case SOLVER_HLLCHC: mx::riemann<DIR, SOLVER_HLLCHC>(ph.gamma, UL, UR, f); break;
+        case SOLVER_ROEHC: mx::riemann<DIR, SOLVER_ROEHC>(ph.gamma, UL, UR, f); break;
         default: mx::riemann<DIR, SOLVER_LLF>(ph.gamma, UL, UR, f); break;
     }
     return true;
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(256) void k_batch_xfix(BatchArgs a, int stage, int 
         for (int m = 0; m < 6; m++)
 #pragma unroll
             for (int v = 0; v < 4; v++) st[m][v] = Y[(size_t)v * g.plane + c0 + (size_t)(m - 2) * g.ld];
-        if (face64<1, ORDER, CHAR>(make_consts(d.ph), d.ph, d.solver, st, f)) {
+        if (face64<1, ORDER, CHAR, true>(make_consts(d.ph), d.ph, d.solver, st, f)) {
             atomicOr(a.faults, 1u);
             atomicOr(&a.stat[k].faults, 1u);
         }
@@ -915,7 +919,7 @@ __global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int k
 #pragma unroll
                     for (int v = 0; v < 4; v++) st[s][v] = bload(Y, d, r - 2 + s, c, v);
                 if (!face_mx<1, ORDER, CHAR>(ph, d.solver, st, faces[m]))
-                    fault |= face64<1, ORDER, CHAR>(kc, ph, d.solver, st, faces[m]);
+                    fault |= face64<1, ORDER, CHAR, true>(kc, ph, d.solver, st, faces[m]);
             } else {
                 const int jf = c - 1 + (m - 2);  // y face between columns jf and jf + 1
 #pragma unroll
@@ -923,7 +927,7 @@ __global__ __launch_bounds__(256) void k_batch_fix(BatchArgs a, int stage, int k
 #pragma unroll
                     for (int v = 0; v < 4; v++) st[s][v] = bload(Y, d, i, jf - 2 + s, v);
                 if (!face_mx<2, ORDER, CHAR>(ph, d.solver, st, faces[m]))
-                    fault |= face64<2, ORDER, CHAR>(kc, ph, d.solver, st, faces[m]);
+                    fault |= face64<2, ORDER, CHAR, true>(kc, ph, d.solver, st, faces[m]);
             }
         }
         const size_t o0 = (size_t)i * g.ld + c;
@@ -974,7 +978,7 @@ __device__ __forceinline__ int sweep_yface(const BProb& d, const Phys& ph, const
         for (int v = 0; v < 4; v++) st[n][v] = Y[(size_t)v * g.plane + c0 + (n - 2)];
     if constexpr (MIXED) {
         if (face_mx<2, ORDER, CHAR>(ph, d.solver, st, out)) return 0;
-        return face64<2, ORDER, CHAR>(make_consts(ph), ph, d.solver, st, out);
+        return face64<2, ORDER, CHAR, true>(make_consts(ph), ph, d.solver, st, out);
     } else {
         return face64<2, ORDER, CHAR, true>(make_consts(ph), ph, d.solver, st, out);
     }

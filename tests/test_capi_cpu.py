@@ -84,8 +84,14 @@ def test_create_rejects_unsupported_forms(lib, scheme, precision, message):
         capi.Solver(cfg)
 
 
-@pytest.mark.parametrize("scheme", [11, 22, 33])
-def test_mixed_batch_rejects_heat_capacity_schemes(lib, scheme):
-    cfgs = [make_config(16, 16, scheme=2, precision=1), make_config(16, 16, scheme=scheme, precision=1)]
-    with pytest.raises(capi.RpfvmError, match="problem 1: the heat-capacity schemes .* FP64 batches only"):
+@pytest.mark.parametrize("key,value,message", [
+    ("precision", 0, "recon_order, characteristic, time_order, precision"),
+    ("time_order", 2, "recon_order, characteristic, time_order, precision"),
+])
+def test_batch_rejects_mixed_schemes_of_integration(lib, key, value, message):
+    """Every problem of a batch shares the reconstruction, time order and precision; a
+    second problem that differs is rejected before any device call."""
+    cfgs = [make_config(16, 16, scheme=2, precision=1), make_config(16, 16, scheme=11, precision=1)]
+    setattr(cfgs[1], key, value)
+    with pytest.raises(capi.RpfvmError, match=message):
         capi.Batch(cfgs)

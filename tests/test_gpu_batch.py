@@ -80,9 +80,9 @@ HC_SCHEMES = (11, 22, 33, 22, 33, 11)
 NARROW_RANGES = {"rho": (0.3, 2.0), "p": (0.2, 2.0), "uv": (-1.0, 1.0)}
 
 
-def hc_batch(characteristic):
-    """small_batch with the heat-capacity solvers HLLHC/HLLCHC/RoeHC (FP64 batches)."""
-    cfgs, states = small_batch(0, characteristic)
+def hc_batch(characteristic, precision=0):
+    """small_batch with the heat-capacity solvers HLLHC/HLLCHC/RoeHC."""
+    cfgs, states = small_batch(precision, characteristic)
     for c, sch in zip(cfgs, HC_SCHEMES):
         c.scheme = sch
     return cfgs, states
@@ -132,9 +132,12 @@ def test_batch_reconstruction_orders_match_single_problems(order, precision, sta
 
 
 @pytest.mark.parametrize("staging", [STAGING_DIRECT, STAGING_LDS, STAGING_SWEEPS])
+@pytest.mark.parametrize("precision", [0, PRECISION_MIXED])
 @pytest.mark.parametrize("characteristic", [True, False])
-def test_batch_heat_capacity_solvers_match_single_problems(characteristic, staging):
-    cfgs, states = hc_batch(characteristic)
+def test_batch_heat_capacity_solvers_match_single_problems(characteristic, precision, staging):
+    """The heat-capacity solvers in FP64 and mixed batches, each problem bit-identical to
+    an rpfvm_ctx of the same form (the mixed batch's FP64 fix-ups included: fault bit 4)."""
+    cfgs, states = hc_batch(characteristic, precision)
     t_end = ORDER_T_END
     with Batch(cfgs) as b:
         b.set_tuning(staging=staging)
