@@ -418,7 +418,8 @@ def noh_metrics(states, n, gamma, t_end=0.6):
 def noh_regions(n, t):
     """Masks of the Noh interior (cell centres) where round-off grows: the origin corner
     r < 0.05, bands of 0.02 along the two reflecting walls, and a band of +-0.01 around the
-    exact shock radius t/3; "rest" is everything else."""
+    exact shock radius t/3; "rest" is everything else, split into the post-shock "plateau"
+    (r < t/3) and the pre-shock "inflow" (r > t/3)."""
     from overheatingproblem_amd import cases
     X, Y = cases.cell_centres(n, n, 1.0 / n, 1.0 / n, ghosts=False)
     r = np.sqrt(X * X + Y * Y)
@@ -426,7 +427,8 @@ def noh_regions(n, t):
     axes = ((X < 0.02) | (Y < 0.02)) & ~corner
     shock = (np.abs(r - t / 3.0) < 0.01) & ~corner & ~axes
     rest = ~(corner | axes | shock)
-    return {"corner": corner, "axes": axes, "shock": shock, "rest": rest}
+    return {"corner": corner, "axes": axes, "shock": shock, "rest": rest,
+            "plateau": rest & (r < t / 3.0), "inflow": rest & (r > t / 3.0)}
 
 
 def region_l1(a, b, masks):
@@ -531,7 +533,8 @@ def noh_full(n, t_end=0.6, every=500):
     Umx, mx_steps, mx_faults, mx_wall = run_own_dts(mx, U0, t_end)
     finals["mixed"] = Umx
     out = {"workload": f"noh-{n}x{n}", "t_end": t_end, "every": every,
-           "regions": "corner r<0.05; axes x<0.02 or y<0.02; shock |r-t/3|<0.01; rest",
+           "regions": "corner r<0.05; axes x<0.02 or y<0.02; shock |r-t/3|<0.01; rest (= plateau "
+                      "r<t/3 + inflow r>t/3)",
            "history": hist}
     m = noh_metrics({"fp64": finals["fp64"], "mixed": finals["mixed"]}, n, cfg.gamma, t_end)
     out.update(m)

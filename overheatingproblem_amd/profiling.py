@@ -169,10 +169,12 @@ def _region_row(label: str, r: dict, regions) -> str:
 
 
 def _history_table(d: dict, runs, regions) -> list:
-    head = "| step | t | " + " | ".join(f"{lab} L1 | {lab} L1 rest" for lab, _ in runs) + " |"
-    lines = [head, "|" + "---:|" * (2 + 2 * len(runs))]
+    cols = ["l1", "l1_rest"] + (["l1_inflow"] if "inflow" in regions else [])
+    names = {"l1": "L1", "l1_rest": "L1 rest", "l1_inflow": "L1 inflow"}
+    head = "| step | t | " + " | ".join(f"{lab} {names[c]}" for lab, _ in runs for c in cols) + " |"
+    lines = [head, "|" + "---:|" * (2 + len(cols) * len(runs))]
     for h in d["history"]:
-        cells = " | ".join(f"{h[k]['l1']:.2e} | {h[k]['l1_rest']:.2e}" for _, k in runs)
+        cells = " | ".join(f"{h[k][c]:.2e}" for _, k in runs for c in cols)
         lines.append(f"| {h['step']} | {h['t']:.4f} | {cells} |")
     return lines
 
@@ -182,6 +184,8 @@ def accuracy_markdown(d: dict, command: str) -> str:
     runs = (("FP64", "fp64"), ("mixed (own dts)", "mixed"), ("mixed (FP64 dts)", "mixed_replay"),
             ("FP64, rho perturbed 1e-12 (FP64 dts)", "fp64_perturbed"))
     regions = ("corner", "axes", "shock", "rest")
+    if "l1_inflow" in d["final_regions"]["mixed"]:
+        regions += ("plateau", "inflow")
     n = d["workload"].split("-")[1]
     lines = [f"# Noh {n} to t = {d['t_end']}: accuracy of the mixed form", "",
              "Command (from the repository root):", "", "```", command, "```", "",
@@ -206,8 +210,8 @@ def accuracy_markdown(d: dict, command: str) -> str:
               " | L-inf |", "|---|---:|" + "---:|" * len(regions) + "---|"]
     for label, k in runs[1:]:
         lines.append(_region_row(label, d["final_regions"][k], regions))
-    lines += ["", "History (FP64 dts, lockstep): L1 over the whole grid and over the rest "
-              "(corner, wall bands and shock band masked).", ""]
+    lines += ["", "History (FP64 dts, lockstep): L1 over the whole grid, over the rest "
+              "(corner, wall bands and shock band masked) and over its pre-shock inflow part.", ""]
     lines += _history_table(d, (("mixed", "mixed_replay"), ("perturbed FP64", "fp64_perturbed")), regions)
     fr = d["final_regions"]
     ratio = fr["mixed_replay"]["l1_rest"] / fr["fp64_perturbed"]["l1_rest"]
@@ -216,6 +220,14 @@ def accuracy_markdown(d: dict, command: str) -> str:
               "FP64 itself travels from a 1e-12 density perturbation "
               f"({fr['fp64_perturbed']['l1_rest']:.2e}); both grow from the origin corner "
               "outwards along the walls and into the post-shock plateau (history)."]
+    if "l1_inflow" in fr["mixed_replay"]:
+        lines += ["", "In the pre-shock inflow (the part of the rest the shock has not reached) "
+                  "the mixed form's L1(rho) is "
+                  f"{fr['mixed_replay']['l1_inflow']:.2e} on the FP64 dts and "
+                  f"{fr['mixed']['l1_inflow']:.2e} on its own (perturbed FP64: "
+                  f"{fr['fp64_perturbed']['l1_inflow']:.2e}); in the post-shock plateau it is "
+                  f"{fr['mixed_replay']['l1_plateau']:.2e} (perturbed FP64 "
+                  f"{fr['fp64_perturbed']['l1_plateau']:.2e})."]
     return "\n".join(lines) + "\n"
 
 
@@ -228,6 +240,10 @@ def accuracy_summary(d: dict) -> dict:
             "l1_rho_fp64_perturbed_1e-12": fr["fp64_perturbed"]["l1"],
             "l1_rho_rest_mixed_fp64_dts": fr["mixed_replay"]["l1_rest"],
             "l1_rho_rest_fp64_perturbed_1e-12": fr["fp64_perturbed"]["l1_rest"],
+            "l1_rho_inflow_mixed_own_dts": fr["mixed"].get("l1_inflow"),
+            "l1_rho_inflow_mixed_fp64_dts": fr["mixed_replay"].get("l1_inflow"),
+            "l1_rho_inflow_fp64_perturbed_1e-12": fr["fp64_perturbed"].get("l1_inflow"),
+            "l1_rho_plateau_mixed_fp64_dts": fr["mixed_replay"].get("l1_plateau"),
             "masked": d["regions"]}
 
 
