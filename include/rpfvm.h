@@ -81,18 +81,23 @@ enum {
 /* rpfvm_config.debug_flags bits. */
 enum {
     RPFVM_DEBUG_MIXED_ALL_FP64 = 1,  /* every mixed-form face through the FP64 recompute list */
-    RPFVM_DEBUG_LARGE_ARRAY_FORMS = 2, /* the kernel forms used when an array (4 planes)
-                                        reaches 4 GB, at any size: the fused y sweep one row
-                                        per wavefront instead of the LDS-staged marching form,
-                                        global instead of buffer addressing in the mixed x
-                                        sweep, the mixed form's two-sweep stage; identical
+    RPFVM_DEBUG_LARGE_ARRAY_FORMS = 2, /* the row forms of the sweeps (the fused y sweep one
+                                        row per wavefront instead of the LDS-staged marching
+                                        form, global instead of buffer addressing, the mixed
+                                        form's two-sweep stage) at any size; they run by
+                                        default where an array (4 planes) reaches 4 GB and on
+                                        grids below 8 M cells (FP64) / 2 M cells (mixed),
+                                        which they fill with more wavefronts; identical
                                         results */
     RPFVM_DEBUG_TWO_SWEEP = 4,       /* the mixed form's stage as an x sweep followed by the
                                         fused y sweep (the default for characteristic
                                         WENO5); identical results */
-    RPFVM_DEBUG_ONE_PASS = 8         /* the mixed form's stage as one pass (x faces, y faces
+    RPFVM_DEBUG_ONE_PASS = 8,        /* the mixed form's stage as one pass (x faces, y faces
                                         and update in one kernel; the default for the other
                                         reconstructions) also for characteristic WENO5;
+                                        identical results */
+    RPFVM_DEBUG_MARCH_FORMS = 16     /* the marching forms on grids below the size where
+                                        the row forms run by default (arrays below 4 GB);
                                         identical results */
 };
 

@@ -27,6 +27,7 @@ DEBUG_MIXED_ALL_FP64 = 1
 DEBUG_LARGE_ARRAY_FORMS = 2
 DEBUG_TWO_SWEEP = 4
 DEBUG_ONE_PASS = 8
+DEBUG_MARCH_FORMS = 16
 
 # The reference's default inflow / fixed-value states: a Mach-3 stream of
 # rho = 1.4, u = 3, p = 1 in conserved form (include/Boundary_Condition.h:49,

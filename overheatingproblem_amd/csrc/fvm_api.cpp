@@ -164,6 +164,19 @@ void rpfvm_config_default(rpfvm_config* c) {
     }
 }
 
+// The row forms of the sweeps where the marching forms cannot run (arrays of 4 GB and
+// more: 32-bit buffer offsets) or lose: a marching wavefront walks 4-8 rows one after
+// the other, so a small grid leaves most SIMDs idle.  Measured on Noh (ms per step through
+// the device loop, marching / row): FP64 200^2 0.267 / 0.141, 1024^2 0.593 / 0.500,
+// 2048^2 1.64 / 1.62, 3072^2 3.25 / 3.40, 4096^2 5.60 / 5.83; mixed 200^2 0.247 / 0.211,
+// 1024^2 0.369 / 0.337, 1280^2 0.474 / 0.457, 1536^2 0.600 / 0.635, 2048^2 0.895 / 0.969.
+static bool row_forms(const rpfvm_config& c, const Geom& g) {
+    if (!buffer_fits(g) || (c.debug_flags & RPFVM_DEBUG_LARGE_ARRAY_FORMS)) return true;
+    if (c.debug_flags & RPFVM_DEBUG_MARCH_FORMS) return false;
+    const double cells = (double)c.nx * (double)c.ny;
+    return cells < (c.precision == RPFVM_PRECISION_MIXED ? 2.0e6 : 8.0e6);
+}
+
 int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     if (!out) return fail(RPFVM_ERR_INVALID, "null output pointer");
     *out = nullptr;
@@ -190,7 +203,7 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ph.source = c.source ? 1 : 0;
     x->ph.positivity = c.positivity ? 1 : 0;
     x->ph.force_fix = (c.debug_flags & RPFVM_DEBUG_MIXED_ALL_FP64) ? 1 : 0;
-    x->ph.large_forms = (c.debug_flags & RPFVM_DEBUG_LARGE_ARRAY_FORMS) ? 1 : 0;
+    x->ph.row_forms = row_forms(c, x->g) ? 1 : 0;
     x->flux = c.precision == RPFVM_PRECISION_MIXED
                   ? select_flux_mixed(c.recon_order, c.characteristic != 0, c.scheme)
                   : select_flux(c.recon_order, c.characteristic != 0, c.scheme);
@@ -330,10 +343,8 @@ int rpfvm_get_state(rpfvm_ctx* x, double* U) {
 // the double-Mach-reflection sides).
 static void enqueue_bc(rpfvm_ctx* x, double* Y, double c) {
     Timed tm(x, RPFVM_KCLASS_BC);
-    for (int s = 0; s < 4; s++) {
-        x->bc[s].c = c;
-        launch_bc_side(Y, x->g, x->bc[s], x->ctl, x->stream);
-    }
+    for (int s = 0; s < 4; s++) x->bc[s].c = c;
+    launch_bc(Y, x->g, x->bc, x->ctl, x->stream);
 }
 
 static void enqueue_fluxes(rpfvm_ctx* x, const double* Y) {
@@ -369,7 +380,7 @@ static void enqueue_dt(rpfvm_ctx* x, const double* clock) {
 // two face computations need a 2-wave register budget in one kernel).  The debug flags
 // force either schedule.
 static bool one_pass(const rpfvm_ctx* x) {
-    if (!x->flux.stage || !use_buffer_forms(x->g, x->ph)) return false;
+    if (!x->flux.stage || !use_march_forms(x->g, x->ph)) return false;
     const int f = x->cfg.debug_flags;
     if (f & RPFVM_DEBUG_TWO_SWEEP) return false;
     if (f & RPFVM_DEBUG_ONE_PASS) return true;

@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64) void k_batch_ghosts(BatchArgs a, int side, int 
         const BcSide s = d.side[side];
         const int t = s.lo + blockIdx.x * blockDim.x + threadIdx.x;
         if (t > s.hi) continue;
-        bc_side_point(a.pool[yi] + d.off, d.g, s, t, blockIdx.y);
+        bc_side_point(a.pool[yi] + d.off, d.g, s, t, blockIdx.y, s.t);  // no time-dependent sides in batches
     }
 }
 

@@ -11,8 +11,9 @@ namespace fvm {
 // owns one (variable, tangential index) pair and writes the four ghost layers in the
 // reference's order, so a side that reads its own ghosts (periodic or reflecting
 // with fewer than four interior cells) sees the same values as the reference loop.
+// `time` is the stage clock (read by the double-Mach-reflection shock side only).
 __device__ __forceinline__ void bc_side_point(double* __restrict__ U, const Geom& g,
-                                              const BcSide& s, int t, int v) {
+                                              const BcSide& s, int t, int v, double time) {
     double* P = U + (size_t)v * g.plane;
     if (s.type == BC_DMR_WALL || s.type == BC_DMR_SHOCK) {
         // bottom / top of BC_DoubleMach_2D (include/Boundary_Condition.h:558-660), row t
@@ -23,7 +24,7 @@ __device__ __forceinline__ void bc_side_point(double* __restrict__ U, const Geom
                 const double x = P[(size_t)t * g.ld + GC + m];
                 P[(size_t)t * g.ld + GC - 1 - m] = xp > s.par[0] ? (v == 2 ? -x : x) : s.value[v];
             } else {
-                const double xs = s.par[0] + (s.par[1] / s.par[2]) + (s.par[3] * s.t / s.par[2]);
+                const double xs = s.par[0] + (s.par[1] / s.par[2]) + (s.par[3] * time / s.par[2]);
                 P[(size_t)t * g.ld + g.C - GC + m] = xp < xs ? s.value[v] : s.alt[v];
             }
         }

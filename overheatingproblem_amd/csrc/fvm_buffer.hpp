@@ -42,8 +42,9 @@ __host__ __device__ inline unsigned long long array_bytes(const Geom& g) {
     return (unsigned long long)g.plane * 4 * sizeof(double);
 }
 __host__ __device__ inline bool buffer_fits(const Geom& g) { return array_bytes(g) < (1ull << 32); }
-// The buffer forms unless the arrays are too large or the caller asked for the others.
-inline bool use_buffer_forms(const Geom& g, const Phys& ph) { return buffer_fits(g) && !ph.large_forms; }
+// The marching, buffer-addressed kernel forms (LDS-staged sweeps, the one-pass stage)
+// unless the context runs the row forms (Phys::row_forms).
+inline bool use_march_forms(const Geom&, const Phys& ph) { return !ph.row_forms; }
 
 // Stencil loads and face stores of the flux kernels for a wavefront on row i, columns
 // j0 + lane: element (v, row/column offset n) of the stencil and the face (v) itself.

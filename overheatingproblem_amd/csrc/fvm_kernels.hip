@@ -27,13 +27,31 @@
 namespace fvm {
 
 // --------------------------------------------------------------- ghost fill
+// The stage clock as the driver forms it: t, t + dt, t + 0.5 dt.
+__device__ __forceinline__ double stage_time(const BcSide& s, const StepCtl* step) {
+    return s.c == 0.0 ? step->t0 : step->t0 + s.c * step->dt;
+}
+
 __global__ void k_bc_side(double* __restrict__ U, Geom g, BcSide s, const StepCtl* __restrict__ step) {
     if (!step->active) return;
     const int t = s.lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (t > s.hi) return;
-    // the stage time as the driver forms it: t, t + dt, t + 0.5 dt
-    s.t = s.c == 0.0 ? step->t0 : step->t0 + s.c * step->dt;
-    bc_side_point(U, g, s, t, blockIdx.y);
+    bc_side_point(U, g, s, t, blockIdx.y, stage_time(s, step));
+}
+
+// Two opposite sides in one launch (blockIdx.y: variable, then side): left + right touch
+// disjoint rows, bottom + top disjoint columns, once the grid has at least 4 interior
+// cells each way; launched in the reference's order L + R, then B + T (which read the
+// corner cells L and R write).
+__global__ void k_bc_pair(double* __restrict__ U, Geom g, BcSide a, BcSide b,
+                          const StepCtl* __restrict__ step) {
+    if (!step->active) return;
+    const bool first = blockIdx.y < 4;
+    const int lo = first ? a.lo : b.lo, hi = first ? a.hi : b.hi;
+    const int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > hi) return;
+    if (first) bc_side_point(U, g, a, t, blockIdx.y, stage_time(a, step));
+    else bc_side_point(U, g, b, t, blockIdx.y - 4, stage_time(b, step));
 }
 
 // ------------------------------------------------------------ face fluxes
@@ -402,7 +420,7 @@ static void launch_flux(const double* Y, double* F, const Geom& g, const Phys& p
     const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
-    if (DIR == 1 && use_buffer_forms(g, ph)) {
+    if (DIR == 1 && use_march_forms(g, ph)) {
         const int rows = g.R - 2 * GC + 1, per = kRingRows * kRingWaves;
         hipLaunchKernelGGL((k_flux_ring<ORDER, CHAR, SOLVER>), dim3((nj + 63) / 64, (rows + per - 1) / per),
                            dim3(64, kRingWaves), 0, s, Y, F, g, ph, faults);
@@ -416,7 +434,7 @@ static void launch_yflux_update(const double* Un, const double* Y, double* out, 
                                 double* G, const Geom& g, const Phys& ph, int kind,
                                 const Repair& rp, const FaceFix&, unsigned, const DtSlots& ds,
                                 unsigned* faults, hipStream_t s) {
-    if (use_buffer_forms(g, ph)) {
+    if (use_march_forms(g, ph)) {
         hipLaunchKernelGGL((k_yflux_march<ORDER, CHAR, SOLVER>), yflux_march_grid(g), yflux_march_block(), 0,
                            s, Un, Y, out, F, G, g, ph, kind, rp, ds, faults);
         return;
@@ -494,6 +512,19 @@ void launch_bc_side(double* U, const Geom& g, const BcSide& s, const StepCtl* st
     dim3 block(64);
     dim3 grid((n + 63) / 64, 4);
     hipLaunchKernelGGL(k_bc_side, grid, block, 0, st, U, g, s, step);
+}
+
+void launch_bc(double* U, const Geom& g, const BcSide side[4], const StepCtl* step, hipStream_t st) {
+    if (g.R - 2 * GC < GC || g.C - 2 * GC < GC) {  // fewer than 4 interior cells: in order
+        for (int k = 0; k < 4; k++) launch_bc_side(U, g, side[k], step, st);
+        return;
+    }
+    for (int k = 0; k < 4; k += 2) {
+        const int n = std::max(side[k].hi - side[k].lo + 1, side[k + 1].hi - side[k + 1].lo + 1);
+        if (n <= 0) continue;
+        hipLaunchKernelGGL(k_bc_pair, dim3((n + 63) / 64, 8), dim3(64), 0, st, U, g, side[k],
+                           side[k + 1], step);
+    }
 }
 
 void launch_update(const double* Un, const double* Y, double* out, const double* F,

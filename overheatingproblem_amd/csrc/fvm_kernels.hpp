@@ -67,7 +67,9 @@ struct Phys {
     double idx, idy;      // 1 / dx, 1 / dy (host-computed; IEEE division rounds alike)
     int source, positivity;
     int force_fix;   // mixed form: send every face to the FP64 recompute list
-    int large_forms;  // the kernel forms for arrays of 4 GB and more, at any size
+    int row_forms;    // the row forms of the sweeps (one row per wavefront, global addressing):
+                      // arrays of 4 GB and more, grids too small to fill the GPU with
+                      // marching wavefronts, or on request
     const StepCtl* step;  // the step's control block (null where no step runs)
 };
 
@@ -168,6 +170,8 @@ FluxPair select_flux_mixed(int order, bool characteristic, int solver);
 void launch_riemann(int dir, int solver, const double* Ls, const double* Rs, double* F,
                     const Geom& g, const Phys& ph, unsigned* faults, hipStream_t s);
 void launch_bc_side(double* U, const Geom& g, const BcSide& s, const StepCtl* step, hipStream_t st);
+// The four sides in the reference's order L, R, B, T (two launches: L + R, then B + T).
+void launch_bc(double* U, const Geom& g, const BcSide side[4], const StepCtl* step, hipStream_t st);
 void launch_update(const double* Un, const double* Y, double* out, const double* F,
                    const double* G, const Geom& g, const Phys& ph, int kind,
                    const Repair& rp, hipStream_t s);

@@ -195,7 +195,7 @@ static void launch_yflux_update_mixed(const double* Un, const double* Y, double*
                                       int kind, const Repair& rp, const FaceFix& fix,
                                       unsigned stamp, const DtSlots& ds, unsigned* faults,
                                       hipStream_t s) {
-    if (use_buffer_forms(g, ph)) {
+    if (use_march_forms(g, ph)) {
         if (ds.slot)
             hipLaunchKernelGGL((k_yflux_march_mixed<ORDER, CHAR, SOLVER, true>), yflux_march_grid(g),
                                yflux_march_block(), 0, s, Un, Y, out, F, G, g, ph, kind, rp, fix, ds);
@@ -220,7 +220,7 @@ static void launch_flux_mixed(const double* Y, double* F, const Geom& g, const P
     const int ni = DIR == 1 ? g.R - 2 * GC + 1 : g.R - 2 * GC;
     dim3 block(64, 4);
     dim3 grid((nj + 63) / 64, (ni + 3) / 4);
-    if (use_buffer_forms(g, ph))
+    if (use_march_forms(g, ph))
         hipLaunchKernelGGL((k_flux_mixed<DIR, ORDER, CHAR, SOLVER, true>), grid, block, 0, s, Y, F, g,
                            ph, fix);
     else
@@ -427,7 +427,7 @@ static void launch_fixup2(const double* Un, const double* Y, double* out, double
                           const Geom& g, const Phys& ph, int kind, const Repair& rp,
                           const FaceFix& fix, unsigned stamp, const DtSlots& ds, unsigned* faults,
                           hipStream_t s) {
-    if (use_buffer_forms(g, ph))
+    if (use_march_forms(g, ph))
         launch_fixup<ORDER, CHAR, SOLVER, YFacesMx<ORDER, CHAR, SOLVER>, XFacesStored>(
             Un, Y, out, F, G, g, ph, kind, rp, fix, stamp, ds, faults, s);
     else

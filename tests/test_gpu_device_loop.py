@@ -8,7 +8,7 @@ import pytest
 
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import FAULT_NONFINITE_DT, PRECISION_MIXED
+from overheatingproblem_amd.config import DEBUG_MARCH_FORMS, FAULT_NONFINITE_DT, PRECISION_MIXED
 
 pytestmark = pytest.mark.gpu
 
@@ -74,12 +74,16 @@ def test_max_steps(steps):
     assert n == steps
 
 
+@pytest.mark.parametrize("forms", ["row", "march"])
 @pytest.mark.parametrize("form", ["fp64", "mixed"])
-def test_safeguard_state_buffers(form):
+def test_safeguard_state_buffers(form, forms):
     """With the safeguard the state trades buffers each step: an odd and an even number
-    of steps taken, and steps enqueued past the end that the device does not take."""
+    of steps taken, and steps enqueued past the end that the device does not take (row
+    forms, the default at this size, and the marching forms)."""
     c, U0 = cases.noh(96, positivity=True)
     c = c if form == "fp64" else mixed(c)
+    if forms == "march":
+        c.debug_flags = DEBUG_MARCH_FORMS
     for t_stop in (0.004, 0.0045, 0.02):
         n, _ = check(c, U0, t_stop, 10 ** 6)
         assert n > 0

@@ -10,7 +10,8 @@ import pytest
 
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import DEBUG_MIXED_ALL_FP64, DEBUG_LARGE_ARRAY_FORMS, PRECISION_MIXED
+from overheatingproblem_amd.config import (DEBUG_LARGE_ARRAY_FORMS, DEBUG_MARCH_FORMS,
+                                           DEBUG_MIXED_ALL_FP64, PRECISION_MIXED)
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -143,14 +144,15 @@ def _row_vs_march_cases():
 
 @pytest.mark.parametrize("name", ["noh", "quad", "wave", "dmr", "grav"])
 def test_buffer_and_large_array_forms_identical(name):
-    """The mixed form's default kernels (fused y sweep in the LDS-staged marching form, x
-    sweep with buffer addressing) and the forms used for arrays of 4 GB and more (one row
-    per wavefront, global addressing; RPFVM_DEBUG_LARGE_ARRAY_FORMS) give the same state,
-    bit for bit, with dt from the device reduction.  The default sweep stores only the y
-    fluxes its fix-up reads and the safeguard recomputes the rest; the large-array sweep
-    stores every one and the safeguard reads them (Noh: repairs in the first steps)."""
+    """The mixed form's marching kernels (fused y sweep in the LDS-staged marching form, x
+    sweep with buffer addressing; RPFVM_DEBUG_MARCH_FORMS at these sizes) and the row forms
+    (one row per wavefront, global addressing; RPFVM_DEBUG_LARGE_ARRAY_FORMS, the default on
+    small grids and for arrays of 4 GB and more) give the same state, bit for bit, with dt
+    from the device reduction.  The marching sweep stores only the y fluxes its fix-up reads
+    and the safeguard recomputes the rest; the row sweep stores every one and the safeguard
+    reads them (Noh: repairs in the first steps)."""
     cfg, U0 = _row_vs_march_cases()[name]
-    c = mixed(cfg)
+    c = mixed(cfg, debug_flags=DEBUG_MARCH_FORMS)
     Um, dm, fm = run(c, U0, steps=12)
     Ur, dr, fr = run(mixed(cfg, debug_flags=DEBUG_LARGE_ARRAY_FORMS), U0, steps=12)
     assert np.array_equal(dm, dr)

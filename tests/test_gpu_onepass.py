@@ -2,23 +2,25 @@
 kernel) against its two-sweep stage (x sweep, then the fused y sweep): the face and cell
 arithmetic is the same, so every state and every dt must be identical, bit for bit, over
 every solver, reconstruction, RK order, the positivity safeguard, the FP64 recompute route
-and grids narrower than one strip or one block of rows.  The two-sweep stage itself is
-checked against the FP64 form in test_gpu_mixed.py."""
+and grids narrower than one strip or one block of rows (the marching forms forced at these
+sizes, where the row forms run by default).  The two-sweep stage itself is checked against
+the FP64 form in test_gpu_mixed.py."""
 import numpy as np
 import pytest
 
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import (DEBUG_MIXED_ALL_FP64, DEBUG_ONE_PASS, DEBUG_TWO_SWEEP,
-                                           PRECISION_MIXED)
+from overheatingproblem_amd.config import (DEBUG_MARCH_FORMS, DEBUG_MIXED_ALL_FP64, DEBUG_ONE_PASS,
+                                           DEBUG_TWO_SWEEP, PRECISION_MIXED)
 
 pytestmark = pytest.mark.gpu
 
 
 def run(cfg, U0, steps, flags):
+    """The marching forms (which the one-pass stage belongs to) at these small sizes."""
     c = cfg.copy()
     c.precision = PRECISION_MIXED
-    c.debug_flags = c.debug_flags | flags
+    c.debug_flags = c.debug_flags | flags | DEBUG_MARCH_FORMS
     with Solver(c) as s:
         s.set_state(U0)
         dts = []

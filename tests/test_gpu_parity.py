@@ -13,7 +13,7 @@ import pytest
 from golden_util import cfg as golden_cfg, keys_with_suffix, load, same
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
-from overheatingproblem_amd.config import DEBUG_LARGE_ARRAY_FORMS
+from overheatingproblem_amd.config import DEBUG_LARGE_ARRAY_FORMS, DEBUG_MARCH_FORMS
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -97,13 +97,17 @@ def test_golden_delta_t(key):
 COMBOS = list(itertools.product((2, 1, 3, 10), (5, 3, 2, 1), (1, 0), (3, 2, 1)))
 
 
+@pytest.mark.parametrize("forms", ["row", "march"])
 @pytest.mark.parametrize("scheme,recon,char,order", COMBOS)
-def test_oracle_matrix(oracle, scheme, recon, char, order):
+def test_oracle_matrix(oracle, scheme, recon, char, order, forms):
     """Every solver x reconstruction x mode x RK order on a non-square grid whose sizes
-    are not multiples of the 64-lane tile, 3 steps with dt from the reduction."""
+    are not multiples of the 64-lane tile, 3 steps with dt from the reduction, in the row
+    forms (the default at this size) and the marching forms (RPFVM_DEBUG_MARCH_FORMS)."""
     c, U0 = cases.four_quadrant(67, 45, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3,
                                 seed=scheme * 100 + recon * 10 + char, scheme=scheme,
                                 recon_order=recon, characteristic=char, time_order=order)
+    if forms == "march":
+        c.debug_flags = DEBUG_MARCH_FORMS
     U = U0.copy()
     ref_dts = oracle.run_steps(c, U, 3)
     with Solver(c) as s:
