@@ -198,6 +198,7 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ph.gm1 = c.gamma - 1.0;
     x->ph.gr = c.gamma / (c.gamma - 1.0);
     x->ph.six = 6.0;
+    x->ph.igm1 = 1.0 / (c.gamma - 1.0);
     x->ph.idx = 1.0 / c.dx;
     x->ph.idy = 1.0 / c.dy;
     x->ph.source = c.source ? 1 : 0;

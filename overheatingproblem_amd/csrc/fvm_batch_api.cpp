@@ -231,6 +231,7 @@ int rpfvm_batch_create(int count, const rpfvm_config* cfgs, rpfvm_batch** out) {
         d.ph.gm1 = c.gamma - 1.0;
         d.ph.gr = c.gamma / (c.gamma - 1.0);
         d.ph.six = 6.0;
+        d.ph.igm1 = 1.0 / (c.gamma - 1.0);
         d.ph.idx = 1.0 / c.dx;
         d.ph.idy = 1.0 / c.dy;
         d.ph.source = c.source ? 1 : 0;

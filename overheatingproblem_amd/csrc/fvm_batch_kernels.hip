@@ -102,7 +102,7 @@ template <int DIR, int ORDER, bool CHAR>
 __device__ __forceinline__ bool face_mx(const Phys& ph, int solver, const double st[6][4],
                                         double f[4]) {
     double UL[4], UR[4];
-    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, ph.igm1, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
     if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
         return false;
     switch (solver) {

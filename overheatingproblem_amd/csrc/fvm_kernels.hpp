@@ -64,6 +64,7 @@ struct StepCtl {
 struct Phys {
     double gamma, dx, dy, cfl, gravity;
     double gm1, gr, six;  // gamma - 1, gamma / (gamma - 1), 6.0 (host-computed)
+    double igm1;          // 1 / (gamma - 1) (host-computed; IEEE division rounds alike)
     double idx, idy;      // 1 / dx, 1 / dy (host-computed; IEEE division rounds alike)
     int source, positivity;
     int force_fix;   // mixed form: send every face to the FP64 recompute list

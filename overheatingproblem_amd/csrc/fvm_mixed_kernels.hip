@@ -76,7 +76,7 @@ __global__ FLUX_MX_BOUNDS void k_flux_mixed(const double* __restrict__ Y,
 #pragma unroll
         for (int v = 0; v < 4; v++) st[n][v] = io.load(Y, v, n - 2, c0, step);
     double UL[4], UR[4], f[4];
-    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
+    mx::reconstruct_face<DIR, ORDER, CHAR>(ph.gamma, ph.igm1, DIR == 1 ? ph.dx : ph.dy, st, UL, UR);
     if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR))) {
         defer_face(fix, DIR - 1, c0);
         // the fused y sweep leaves the cells of a NaN x face to the fix-up
@@ -129,7 +129,7 @@ struct FaceOpMx {
     FaceFix fix;
     __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
         double UL[4], UR[4];
-        mx::reconstruct_face<2, ORDER, CHAR>(ph.gamma, ph.dy, st, UL, UR);
+        mx::reconstruct_face<2, ORDER, CHAR>(ph.gamma, ph.igm1, ph.dy, st, UL, UR);
         if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
             return false;
         mx::riemann<2, SOLVER>(ph.gamma, UL, UR, f);
@@ -145,7 +145,7 @@ struct FaceOpMxX {
     FaceFix fix;
     __device__ __forceinline__ bool compute(const double st[6][4], double f[4]) {
         double UL[4], UR[4];
-        mx::reconstruct_face<1, ORDER, CHAR>(ph.gamma, ph.dx, st, UL, UR);
+        mx::reconstruct_face<1, ORDER, CHAR>(ph.gamma, ph.igm1, ph.dx, st, UL, UR);
         if (ph.force_fix || !(mx::admissible64(ph.gamma, UL) && mx::admissible64(ph.gamma, UR)))
             return false;
         mx::riemann<1, SOLVER>(ph.gamma, UL, UR, f);
