@@ -54,30 +54,6 @@ __global__ void k_bc_pair(double* __restrict__ U, Geom g, BcSide a, BcSide b,
     else bc_side_point(U, g, b, t, blockIdx.y - 4, stage_time(b, step));
 }
 
-// The four sides in one workgroup, for grids whose sides are short: left + right, a
-// barrier, then bottom + top (the order of the two-launch form).  Each launch between two
-// dependent kernels costs a few microseconds of GPU time, most of a small grid's ghost fill.
-__global__ __launch_bounds__(1024) void k_bc_block(double* __restrict__ U, Geom g, BcSide l,
-                                                   BcSide r, BcSide b, BcSide t,
-                                                   const StepCtl* __restrict__ step) {
-    if (!step->active) return;
-#pragma unroll 1
-    for (int pair = 0; pair < 2; pair++) {
-        const BcSide& a = pair ? b : l;
-        const BcSide& c = pair ? t : r;
-        const int na = a.hi - a.lo + 1, nc = c.hi - c.lo + 1;
-        const int n = 4 * (max(na, 0) + max(nc, 0));
-        for (int q = threadIdx.x; q < n; q += blockDim.x) {
-            const bool first = q < 4 * na;
-            const int u = first ? q : q - 4 * na;
-            const int m = first ? na : nc;
-            const BcSide& s = first ? a : c;
-            bc_side_point(U, g, s, s.lo + u % m, u / m, stage_time(s, step));
-        }
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------ face fluxes
 // One thread per face.  DIR 1: faces (i+1/2, j), i = GC-1..R-GC-1, j = GC..C-GC-1;
 // DIR 2: faces (i, j+1/2), i = GC..R-GC-1, j = GC-1..C-GC-1.  The flux is stored at
@@ -538,19 +514,9 @@ void launch_bc_side(double* U, const Geom& g, const BcSide& s, const StepCtl* st
     hipLaunchKernelGGL(k_bc_side, grid, block, 0, st, U, g, s, step);
 }
 
-#ifndef RPFVM_BC_BLOCK_MAX
-#define RPFVM_BC_BLOCK_MAX 1100
-#endif
-constexpr int kBcBlockMax = RPFVM_BC_BLOCK_MAX;  // rows / columns (with ghosts) of k_bc_block
-
 void launch_bc(double* U, const Geom& g, const BcSide side[4], const StepCtl* step, hipStream_t st) {
     if (g.R - 2 * GC < GC || g.C - 2 * GC < GC) {  // fewer than 4 interior cells: in order
         for (int k = 0; k < 4; k++) launch_bc_side(U, g, side[k], step, st);
-        return;
-    }
-    if (std::max(g.R, g.C) <= kBcBlockMax) {  // short sides: one workgroup, one launch
-        hipLaunchKernelGGL(k_bc_block, dim3(1), dim3(1024), 0, st, U, g, side[0], side[1], side[2],
-                           side[3], step);
         return;
     }
     for (int k = 0; k < 4; k += 2) {
