@@ -738,8 +738,8 @@ def main():
                          "wall-heating report instead")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has "
-              f"no multi-GPU path; see DESIGN.md for how the step would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library has "
+              f"no multi-GPU path; it runs one GPU's share of the step)", file=sys.stderr)
         sys.exit(2)
 
     if args.noh_full:

@@ -1,7 +1,8 @@
 // fvm_kernels.hip -- gfx950 kernels of the FP64 finite-volume step.
 //
 // Per RK stage (include/time_advance.h:823-872):
-//   k_bc_side  x4   ghost fill of the stage input, sides L, R, B, T in order
+//   k_bc_pair  x2   ghost fill of the stage input: sides L + R, then B + T (k_bc_side x4,
+//                   one side per launch in order, on grids under 4 interior cells a side)
 //   k_flux<1>       x-face fluxes  F[i+1/2][j]: 6-cell stencil -> reconstruction -> Riemann
 //   k_flux<2>       y-face fluxes  G[i][j+1/2]
 //   k_update        L = -(dF)/dx - (dG)/dy + S and the SSP-RK stage combine
