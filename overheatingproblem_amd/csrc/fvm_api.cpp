@@ -164,6 +164,8 @@ void rpfvm_config_default(rpfvm_config* c) {
     }
 }
 
+static bool one_pass(const rpfvm_ctx* x);
+
 // The row forms of the sweeps where the marching forms cannot run (arrays of 4 GB and
 // more: 32-bit buffer offsets) or lose: a marching wavefront walks 4-8 rows one after
 // the other, so a small grid leaves most SIMDs idle.  Measured on Noh (ms per step through
@@ -250,7 +252,8 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     const size_t fix_cap = (size_t)(c.nx + 2 + (c.nx + 3) / 4) * (size_t)(c.ny + 2 + (c.ny + kStrip - 1) / kStrip);
     const size_t face_bytes = ((fix_cap * 4 + 255) / 256) * 256;
     const size_t fix_bytes = c.precision == RPFVM_PRECISION_MIXED ? 2 * face_bytes + 256 : 0;
-    const size_t slot_bytes = (((size_t)dt_slot_count(x->g) * 2 * sizeof(unsigned long long) + 255) / 256) * 256;
+    // slots for one row per slot, the most any form uses
+    const size_t slot_bytes = (((size_t)dt_slot_count(x->g, 1) * 2 * sizeof(unsigned long long) + 255) / 256) * 256;
     const size_t ctl_bytes = 256;
     const size_t total = narr * arr_bytes + small + rep_bytes + fix_bytes + slot_bytes + ctl_bytes;
     he = hipMalloc(&x->pool, total);
@@ -286,6 +289,10 @@ int rpfvm_create(const rpfvm_config* cfg, rpfvm_ctx** out) {
     x->ctl = reinterpret_cast<StepCtl*>(q + small + rep_bytes + fix_bytes + slot_bytes);
     x->ph.step = x->ctl;
     x->slots.nstrips = (c.ny + kStrip - 1) / kStrip;
+    // rows per slot: those a wavefront of the marching fused y sweep walks, which reduces its
+    // lanes' maxima once; one row in the row forms and in the one-pass stage (whose 16-row
+    // wavefronts reduce per row: carrying the maxima across its rows spills registers)
+    x->slots.rows = use_march_forms(x->g, x->ph) && !one_pass(x) ? march_rows() : 1;
     he = hipMemsetAsync(x->pool, 0, total, x->stream);
     if (he != hipSuccess) return cleanup(fail(RPFVM_ERR_HIP, "hipMemset: %s", hipGetErrorString(he)));
     he = hipHostMalloc(reinterpret_cast<void**>(&x->host_scal), 256, hipHostMallocDefault);
@@ -400,7 +407,7 @@ static bool enqueue_stage(rpfvm_ctx* x, const double* Un, double* Y, double* out
     const unsigned stamp = ++x->stamp;
     enqueue_bc(x, Y, c);
     const bool fused = out != Y;
-    const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips};
+    const DtSlots ds{last && fused ? x->slots.slot : nullptr, x->slots.nstrips, x->slots.rows};
     const bool op = fused && one_pass(x);
     if (op) {
         Timed t(x, RPFVM_KCLASS_FLUX_Y);

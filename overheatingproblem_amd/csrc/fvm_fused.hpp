@@ -46,12 +46,9 @@ __device__ __forceinline__ void cell_speeds(double gam, double rho, double ru, d
     ly = fabs(v) + a;
 }
 __device__ __forceinline__ double nanmax(double a, double b) { return b > a ? b : a; }
-// Maximum over the wavefront of lane values that are +0 or positive, never NaN or -0 (each
-// is a running "if (s > S) S = s" from S = +0, or a maximum of such): there fmax is the
-// same maximum as nanmax in one instruction instead of a compare and two selects.
 __device__ __forceinline__ double wave_max(double x) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    for (int o = 32; o > 0; o >>= 1) x = nanmax(x, __shfl_xor(x, o, 64));
     return x;
 }
 
@@ -258,6 +255,7 @@ __device__ __forceinline__ void yflux_update_body(const double* Un, const double
         lx = wave_max(lx);
         ly = wave_max(ly);
         if (t == 0 && i < g.R - GC) {
+            // one row per wavefront: ds.rows == 1
             const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((f + 1 - GC) / kStrip);
             ds.slot[2 * w] = (unsigned long long)__double_as_longlong(lx);
             ds.slot[2 * w + 1] = (unsigned long long)__double_as_longlong(ly);
@@ -373,6 +371,7 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
     fetch_window(i0, 0);
     fetch_rows(i0);
     wait_vmem();
+    double lx = 0.0, ly = 0.0;  // the lane's wave-speed maxima over its rows (last stage)
     for (int row = i0; row < i1; row++) {
         // Row and lane indices pass through empty asm statements, so that the compiler
         // forms each store address afresh instead of carrying ~20 per-lane 64-bit
@@ -415,7 +414,6 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
 #pragma unroll
             for (int v = 0; v < 4; v++) buffer_store_f64(gf[v], rg, fb, (int)(rowb + v * pl8), 0);
         }
-        double lx = 0.0, ly = 0.0;
         if (cell_ok) {
             double y[4], un[4], fu[4], W[4], R[4];
 #pragma unroll
@@ -444,17 +442,17 @@ __device__ __forceinline__ void yflux_update_march(const double* Un, const doubl
 #pragma unroll
             for (int v = 0; v < 4; v++) fl[v] = fu[v];
         }
-        if (ds.slot) {
-            lx = wave_max(lx);
-            ly = wave_max(ly);
-            if (t == 0) {
-                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)strip;
-                ds.slot[2 * sw] = (unsigned long long)__double_as_longlong(lx);
-                ds.slot[2 * sw + 1] = (unsigned long long)__double_as_longlong(ly);
-            }
-        }
         wait_lds();  // this row's reads of the window and the F / U^n rows are complete
         if (i + 1 < i1) fetch_rows(i + 1);
+    }
+    if (ds.slot) {  // one slot for the wavefront's rows (ds.rows == kMarchRows)
+        lx = wave_max(lx);
+        ly = wave_max(ly);
+        if (t == 0) {
+            const size_t sw = (size_t)((i0 - GC) / ds.rows) * ds.nstrips + (size_t)strip;
+            ds.slot[2 * sw] = (unsigned long long)__double_as_longlong(lx);
+            ds.slot[2 * sw + 1] = (unsigned long long)__double_as_longlong(ly);
+        }
     }
 }
 

@@ -333,15 +333,15 @@ __global__ __launch_bounds__(256) void k_dt_partial(const double* __restrict__ U
 }
 
 // The same partial maxima from the slots of the last stage.  Each wavefront takes 64
-// consecutive slots; a stale one (kSlotStale) has its row segment of up to 63 cells
-// recomputed from U by the whole wavefront, one cell per lane.
+// consecutive slots; a stale one (kSlotStale) has its cells (up to 63 columns of ds.rows
+// rows) recomputed from U by the whole wavefront, one column per lane.
 __global__ __launch_bounds__(256) void k_dt_slots(const double* __restrict__ U, Geom g, Phys ph,
                                                   DtSlots ds, double* __restrict__ partial) {
     const double gam = ph.gamma;
     double sx = 0.0, sy = 0.0;
     const int nj = g.C - 2 * GC;
     const int lane = threadIdx.x & 63;
-    const long total = (long)(g.R - 2 * GC) * ds.nstrips;
+    const long total = (long)dt_slot_count(g, ds.rows);
     const long waves = (long)gridDim.x * (blockDim.x >> 6);
     for (long base = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < total;
          base += waves * 64) {
@@ -357,14 +357,17 @@ __global__ __launch_bounds__(256) void k_dt_slots(const double* __restrict__ U, 
         }
         for (unsigned long long m = __ballot(stale); m; m &= m - 1) {
             const long ws = base + (__ffsll((long long)m) - 1);
-            const int i = GC + (int)(ws / ds.nstrips);
+            const int ia = GC + (int)(ws / ds.nstrips) * ds.rows;
+            const int ib = min(ia + ds.rows, g.R - GC);
             const int j = (int)(ws % ds.nstrips) * kStrip + lane;
             if (lane < kStrip && j < nj) {
-                const size_t c = (size_t)i * g.ld + GC + j;
-                double lx, ly;
-                cell_speeds(gam, U[c], U[g.plane + c], U[2 * g.plane + c], U[3 * g.plane + c], lx, ly);
-                if (lx > sx) sx = lx;
-                if (ly > sy) sy = ly;
+                for (int i = ia; i < ib; i++) {
+                    const size_t c = (size_t)i * g.ld + GC + j;
+                    double lx, ly;
+                    cell_speeds(gam, U[c], U[g.plane + c], U[2 * g.plane + c], U[3 * g.plane + c], lx, ly);
+                    if (lx > sx) sx = lx;
+                    if (ly > sy) sy = ly;
+                }
             }
         }
     }
@@ -616,14 +619,12 @@ int dt_blocks(const Geom& g) {
     return (int)(b < 2048 ? (b < 1 ? 1 : b) : 2048);
 }
 
-int dt_slot_count(const Geom& g) {
-    return (g.R - 2 * GC) * ((g.C - 2 * GC + kStrip - 1) / kStrip);
-}
+int march_rows() { return kMarchRows; }
 
 void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
                      double* partial, double* dt_out, const double* clock, unsigned* faults,
                      hipStream_t s) {
-    const long slots = dt_slot_count(g);
+    const long slots = dt_slot_count(g, ds.rows);
     const int nb = (int)std::min<long>(dt_blocks(g), std::max<long>(1, (slots + 255) / 256));
     hipLaunchKernelGGL(k_dt_slots, dim3(nb), dim3(256), 0, s, U, g, ph, ds, partial);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, s, partial, nb, ph, dt_out, clock, faults);
@@ -632,7 +633,7 @@ void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlo
 void launch_dt_loop(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
                     double* partial, StepCtl* step, unsigned* faults, hipStream_t s) {
     if (ds.slot) {
-        const long slots = dt_slot_count(g);
+        const long slots = dt_slot_count(g, ds.rows);
         const int nb = (int)std::min<long>(dt_blocks(g), std::max<long>(1, (slots + 255) / 256));
         hipLaunchKernelGGL(k_dt_slots, dim3(nb), dim3(256), 0, s, U, g, ph, ds, partial);
         hipLaunchKernelGGL(k_dt_loop, dim3(1), dim3(64), 0, s, partial, nb, ph, step, faults);

@@ -118,10 +118,13 @@ struct FaceFix {
     unsigned cap;   // entries per list (every face, plus the faces two wavefronts share)
 };
 
-// Wave-speed maxima of the last stage's new state, one slot per wavefront row of the fused
-// y-flux kernel: slot w = (i - GC) * nstrips + (j - GC) / 63 holds the bit patterns of
-// max(|u|+a) and max(|v|+a) over cells (i, GC + 63 s .. GC + 63 s + 62) (non-negative
-// doubles order as unsigned integers).  A kernel that changes such a cell afterwards
+// Wave-speed maxima of the last stage's new state, one slot per wavefront of the fused
+// y-flux kernel: slot w = ((i - GC) / rows) * nstrips + (j - GC) / 63 holds the bit
+// patterns of max(|u|+a) and max(|v|+a) over the cells of strip s = w % nstrips (columns
+// GC + 63 s .. GC + 63 s + 62) in the `rows` rows from GC + rows (w / nstrips) (non-negative
+// doubles order as unsigned integers).  rows: the rows a wavefront of the marching fused y
+// sweep walks (which then reduces its lanes' maxima once), 1 for the row forms and the
+// one-pass stage.  A kernel that changes such a cell afterwards
 // marks its slot stale with the all-ones pattern; the reduction recomputes stale slots
 // from the state.  slot == nullptr: not the last stage.
 constexpr int kStrip = 63;  // cells updated per 64-lane row of the fused y-flux kernels
@@ -129,6 +132,7 @@ constexpr int kStrip = 63;  // cells updated per 64-lane row of the fused y-flux
 struct DtSlots {
     unsigned long long* slot;
     int nstrips;
+    int rows;  // grid rows per slot
 };
 constexpr unsigned long long kSlotStale = ~0ull;
 
@@ -197,6 +201,9 @@ void launch_dt(const double* U, const Geom& g, const Phys& ph, double* partial, 
 void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
                      double* partial, double* dt_out, const double* clock, unsigned* faults,
                      hipStream_t s);
-int dt_slot_count(const Geom& g);
+__host__ __device__ inline int dt_slot_count(const Geom& g, int rows) {
+    return ((g.R - 2 * GC + rows - 1) / rows) * ((g.C - 2 * GC + kStrip - 1) / kStrip);
+}
+int march_rows();  // rows a wavefront of the marching fused y sweep walks
 
 }  // namespace fvm

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_fixcells_mx(const double* Un, const do
         }
         const bool ok = cell_update(Un, Y, out, F, G, g, ph, dt, kind, i, j);
         if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
-            const size_t w = (size_t)(i - GC) * ds.nstrips + (size_t)((j - GC) / kStrip);
+            const size_t w = (size_t)((i - GC) / ds.rows) * ds.nstrips + (size_t)((j - GC) / kStrip);
             ds.slot[2 * w] = kSlotStale;
         }
         if (rp.cnt && !ok) {

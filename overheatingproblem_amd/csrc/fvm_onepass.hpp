@@ -224,7 +224,7 @@ __device__ __forceinline__ void stage_march(const double* Un, const double* Y, d
             lx = wave_max(lx);
             ly = wave_max(ly);
             if (t == 0) {
-                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)strip;
+                const size_t sw = (size_t)(i - GC) * ds.nstrips + (size_t)strip;  // ds.rows == 1
                 ds.slot[2 * sw] = (unsigned long long)__double_as_longlong(lx);
                 ds.slot[2 * sw + 1] = (unsigned long long)__double_as_longlong(ly);
             }

@@ -127,7 +127,7 @@ __device__ __forceinline__ void repair_cells_phase(const double* Un, const doubl
             ok = admissible(Consts{ph.gamma, ph.gm1, ph.gr, Recip{}}, W);
         }
         if (ds.slot) {  // the last stage's wave-speed slot of this cell is stale now
-            const size_t w = (size_t)(c / rp.ny) * ds.nstrips + (size_t)((c % rp.ny) / kStrip);
+            const size_t w = (size_t)((c / rp.ny) / ds.rows) * ds.nstrips + (size_t)((c % rp.ny) / kStrip);
             ds.slot[2 * w] = kSlotStale;
         }
         if (!ok) {
