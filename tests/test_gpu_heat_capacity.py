@@ -11,6 +11,7 @@ import pytest
 from golden_util import cfg as golden_cfg, keys_with_suffix, load, same
 from overheatingproblem_amd import cases
 from overheatingproblem_amd.capi import Solver
+from overheatingproblem_amd.config import DEBUG_MARCH_FORMS
 
 pytestmark = pytest.mark.gpu
 GC = 4
@@ -93,13 +94,17 @@ MATRIX = [(s, r, ch, o) for s in (11, 22, 33) for r in (5, 3, 2, 1) for ch in (1
           for o in ((3, 2, 1) if r == 5 else (3,))]
 
 
+@pytest.mark.parametrize("forms", ["row", "march"])
 @pytest.mark.parametrize("scheme,recon,char,order", MATRIX)
-def test_heat_capacity_oracle_matrix(oracle, scheme, recon, char, order):
+def test_heat_capacity_oracle_matrix(oracle, scheme, recon, char, order, forms):
     """Every heat-capacity solver x reconstruction x mode (and RK order for WENO5) on a
-    non-square grid whose sizes are not multiples of the 64-lane tile, 3 steps."""
+    non-square grid whose sizes are not multiples of the 64-lane tile, 3 steps, in the row
+    forms (the default at this size) and the marching forms."""
     c, U0 = cases.four_quadrant(67, 45, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3,
                                 seed=scheme * 100 + recon * 10 + char, scheme=scheme,
                                 recon_order=recon, characteristic=char, time_order=order)
+    if forms == "march":
+        c.debug_flags = DEBUG_MARCH_FORMS
     U = U0.copy()
     ref_dts = oracle.run_steps(c, U, 3)
     with Solver(c) as s:

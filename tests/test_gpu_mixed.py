@@ -43,16 +43,18 @@ def rel_l1(a, b):
     return float(np.abs(a - b).mean() / np.abs(b).mean())
 
 
+@pytest.mark.parametrize("forms", [0, DEBUG_MARCH_FORMS])
 @pytest.mark.parametrize("scheme,recon,char", [(2, 5, 1), (3, 5, 0), (1, 2, 1), (10, 3, 1),
                                                (2, 1, 0), (3, 3, 0), (11, 5, 1), (22, 5, 1),
                                                (33, 5, 0), (22, 2, 0)])
-def test_fp64_recompute_route_is_exact(scheme, recon, char):
+def test_fp64_recompute_route_is_exact(scheme, recon, char, forms):
     """With every face sent through the mixed form's FP64 recompute list, the result is
-    the FP64 path's, bit for bit (tests the list/fix-up machinery)."""
+    the FP64 path's, bit for bit (tests the list/fix-up machinery), in the row forms (the
+    default at this size) and the marching forms (two sweeps or one pass by order)."""
     c, U0 = cases.four_quadrant(70, 51, cases.LAX_LIU_3, x0=0.8, y0=0.8, perturb=1e-3, seed=9,
                                 scheme=scheme, recon_order=recon, characteristic=char)
     U64, dts, _ = run(c, U0, steps=3)
-    Umx, _, faults = run(mixed(c, debug_flags=DEBUG_MIXED_ALL_FP64), U0, dts=dts)
+    Umx, _, faults = run(mixed(c, debug_flags=DEBUG_MIXED_ALL_FP64 | forms), U0, dts=dts)
     assert faults & 4  # the fallback was used
     assert np.array_equal(U64, Umx)
 
