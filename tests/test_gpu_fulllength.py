@@ -11,7 +11,8 @@ t_end, then RK3_TimeAd, include/time_advance.h:788-882):
     t = 0.3.
 
 The device must reproduce every dt and the final state bit for bit (a zero's sign
-aside).  On a mismatch the test reports the first step whose dt differs and the
+aside), in the row forms of the sweeps (the default at these sizes) and in the marching
+forms (the default from 8 M cells).  On a mismatch the test reports the first step whose dt differs and the
 per-variable sums and sampled cells of the final interior.
 """
 import numpy as np
@@ -58,11 +59,15 @@ def explain(d, name, U, dts):
             f"differing: {int((samp != d[f'{name}_sample']).sum())} of {samp.size}")
 
 
+@pytest.mark.parametrize("forms", ["row", "march"])
 @pytest.mark.parametrize("name", RUNS)
-def test_full_run_bit_identical_to_reference(name):
+def test_full_run_bit_identical_to_reference(name, forms):
     from overheatingproblem_amd.capi import Solver
+    from overheatingproblem_amd.config import DEBUG_MARCH_FORMS
     d = golden.load("fulllength")
     cfg = golden.cfg(d, f"{name}_cfg")
+    if forms == "march":
+        cfg.debug_flags = DEBUG_MARCH_FORMS
     t_end = float(d[f"{name}_t_end"])
     ref_dts = d[f"{name}_dts"]
     print(f"{name}: {cfg.nx}x{cfg.ny} to t={t_end}, {len(ref_dts)} reference steps", flush=True)
@@ -77,12 +82,16 @@ def test_full_run_bit_identical_to_reference(name):
     assert ok, explain(d, name, U, dts)
 
 
-def test_device_loop_matches_reference_sedov():
+@pytest.mark.parametrize("forms", ["row", "march"])
+def test_device_loop_matches_reference_sedov(forms):
     """rpfvm_advance_to (dt and clock on the device) over the whole Sedov run."""
     from overheatingproblem_amd.capi import Solver
+    from overheatingproblem_amd.config import DEBUG_MARCH_FORMS
     d = golden.load("fulllength")
     name = "sedov200"
     cfg = golden.cfg(d, f"{name}_cfg")
+    if forms == "march":
+        cfg.debug_flags = DEBUG_MARCH_FORMS
     with Solver(cfg) as s:
         s.set_state(d[f"{name}_U0"])
         t, steps = s.advance_to(float(d[f"{name}_t_end"]), 10 ** 6)
