@@ -489,11 +489,15 @@ __global__ __launch_bounds__(64) void k_batch_ghosts(BatchArgs a, int side, int 
 // The x faces F[r][c] (between rows r and r + 1) of every active strip: lane t owns
 // column c = j0 + t (t < 63) and marches the strip's rows, as one row of k_flux<1> per
 // step (the first strip row of a problem also computes its lower boundary face).
-// Mixed form: a face whose FP32 states are not admissible goes to xdefer_list.  BUF: loads
-// and stores through buffer descriptors (row offsets in SGPRs, the lane's column in one
-// VGPR; every problem's arrays below 4 GB), otherwise global addresses.
+// Mixed form: a face whose FP32 states are not admissible goes to xdefer_list.  BUF: the
+// stencil rows staged through an LDS ring by direct buffer loads, one row ahead of the face
+// arithmetic, and the fluxes stored through a buffer descriptor (row offsets in SGPRs, the
+// lane's column in one VGPR; every problem's arrays below 4 GB); otherwise global loads of
+// the 24 stencil values per face.  A row walk that loads each face's stencil just before
+// its arithmetic left 30% of the VALU issue idle (0.68 busy at 3.9 waves/SIMD).
 template <int ORDER, bool CHAR, bool MIXED, bool BUF>
-__global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int yi) {
+__global__ __launch_bounds__(64, (MIXED || !BUF ? 1 : 3)) void k_batch_xsweep(BatchArgs a, int stage, int yi) {
+    __shared__ double xring[BUF ? 6 * 4 * 64 : 1];  // 12 KB: six stencil rows of the strip
     const int lane = threadIdx.x;
     const int nact = a.sched->nact, total = a.sched->total;
     unsigned* queue = &a.sched->xqueue[stage];
@@ -531,19 +535,49 @@ __global__ __launch_bounds__(64) void k_batch_xsweep(BatchArgs a, int stage, int
             cb0 = (unsigned)(GC + sy * kStripCols) * 8u;  // the strip's first column
         }
         const int lb = lane * 8;
-        for (int r = i0 == GC ? GC - 1 : i0; r < i1; r++) {
-            if (!col_ok) continue;
+        const int r_first = i0 == GC ? GC - 1 : i0;
+        // BUF: the stencil rows r_first-2 .. i1+2 of the strip (64 columns, 4 variables each)
+        // pass through a ring of 6 LDS slots, row q in slot q mod 6, loaded straight into
+        // LDS (no VGPR holds them); row r + 4 is requested as soon as face row r has read
+        // the slot of row r - 2, so it arrives during face row r's arithmetic.  Lanes past
+        // the row's last column load its last dword again (never read).  Rows stay within
+        // 1 .. R-2 (r_first >= GC - 1, i1 <= R - GC).
+        int off0 = 0, off1 = 0;
+        if constexpr (BUF) {
+            const int cw = 2 * min(64, g.C - (GC + sy * kStripCols));  // valid dwords of a row
+            off0 = 4 * min(lane, cw - 1);
+            off1 = 4 * min(64 + lane, cw - 1);
+        }
+        auto fetch_row = [&](int q) {
+            const unsigned o = (unsigned)q * ld8 + cb0;
+            unsigned* slot = reinterpret_cast<unsigned*>(xring + (q % 6) * 4 * 64);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                buffer_load_lds_dword(ry, (lds_void*)(slot + v * 128), 4, off0, (int)(o + v * pl8), 0, 0);
+                buffer_load_lds_dword(ry, (lds_void*)(slot + v * 128 + 64), 4, off1, (int)(o + v * pl8), 0, 0);
+            }
+        };
+        if constexpr (BUF) {
+            for (int q = r_first - 2; q <= r_first + 3; q++) fetch_row(q);
+        }
+        for (int r = r_first; r < i1; r++) {
             double st[6][4], f[4];
             size_t c0 = 0, pl = 0;
             unsigned rb = 0;
             if constexpr (BUF) {
+                wait_vmem();  // rows up to r + 3 (and the stores of face row r - 1)
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    const double* slot = xring + ((r - 2 + n) % 6) * 4 * 64;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) st[n][v] = slot[v * 64 + lane];
+                }
+                wait_lds();  // the slot of row r - 2 is read for the last time
+                if (r + 1 < i1) fetch_row(r + 4);
+                if (!col_ok) continue;
                 rb = (unsigned)opaque_row(r) * ld8 + cb0;
-#pragma unroll
-                for (int n = 0; n < 6; n++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++)
-                        st[n][v] = buffer_load_f64(ry, lb, (int)(rb + v * pl8 + (unsigned)(n - 2) * ld8), 0);
             } else {
+                if (!col_ok) continue;
                 // strides through an opaque copy, so that the 24 stencil offsets are formed
                 // at their loads instead of hoisted out of the row loop into live registers
                 size_t ld = g.ld;
