@@ -748,12 +748,16 @@ __global__ __launch_bounds__(64) void k_batch_ysweep(BatchArgs a, int stage, int
 // before the current row's face arithmetic and waited for after it; the lower x-flux row
 // is carried in registers.  Same arithmetic and results as k_batch_ysweep.  Used when
 // every problem's arrays are below 4 GB (32-bit buffer offsets, BatchArgs::buf_ok).
-// Waves/SIMD: mixed 4 (126 VGPRs); FP64 3 (168 VGPRs, 4 spilled; unbounded it takes 179).
+// Waves/SIMD: mixed 4 (128 VGPRs, 3 dwords spilled; bounded to 3 it does not spill and the
+// batch runs 2% slower); FP64 3 (168 VGPRs, 4 spilled; unbounded it takes 179).
+#ifndef RPFVM_BATCH_MIXED_WAVES
+#define RPFVM_BATCH_MIXED_WAVES 4
+#endif
 #ifndef RPFVM_BATCH_FP64_WAVES
 #define RPFVM_BATCH_FP64_WAVES 3
 #endif
 template <int ORDER, bool CHAR, bool MIXED>
-__global__ __launch_bounds__(64, MIXED ? 4 : RPFVM_BATCH_FP64_WAVES) void k_batch_ysweep_lds(BatchArgs a, int stage, int kind,
+__global__ __launch_bounds__(64, MIXED ? RPFVM_BATCH_MIXED_WAVES : RPFVM_BATCH_FP64_WAVES) void k_batch_ysweep_lds(BatchArgs a, int stage, int kind,
                                                             int yi, int oi) {
     __shared__ double lds[kMarchLds];
     double* const win = lds;                      // [2][4][kWinPitch]
