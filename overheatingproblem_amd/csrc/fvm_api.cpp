@@ -320,18 +320,31 @@ int rpfvm_destroy(rpfvm_ctx* x) {
     return 0;
 }
 
+// Host arrays in the reference layout [4][R][C] cross PCIe as one contiguous copy into (or
+// out of) a state-sized scratch array, re-pitched to the device rows of ld doubles by a
+// kernel: a pitched host copy moves row by row (32 KB rows at 4096^2) at 3.3 GB/s.  The
+// scratch is U1 (L where U1 or U2 is the copy's own end): stage arrays whose contents no
+// caller reads between steps.
+static double* copy_stage(rpfvm_ctx* x, const double* dev) {
+    return dev == x->U1 || dev == x->U2 ? x->L : x->U1;
+}
+
 static int copy_in(rpfvm_ctx* x, double* dev, const double* host) {
-    HIPCHK(hipMemcpy2DAsync(dev, x->g.ld * sizeof(double), host, x->g.C * sizeof(double),
-                            x->g.C * sizeof(double), 4 * (size_t)x->g.R, hipMemcpyHostToDevice,
-                            x->stream));
+    const size_t C = (size_t)x->g.C, rows = 4 * (size_t)x->g.R;
+    double* stage = copy_stage(x, dev);
+    HIPCHK(hipMemcpyAsync(stage, host, rows * C * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    launch_repitch(stage, C, dev, (size_t)x->g.ld, rows, C, x->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
 }
 
 static int copy_out(rpfvm_ctx* x, double* host, const double* dev) {
-    HIPCHK(hipMemcpy2DAsync(host, x->g.C * sizeof(double), dev, x->g.ld * sizeof(double),
-                            x->g.C * sizeof(double), 4 * (size_t)x->g.R, hipMemcpyDeviceToHost,
-                            x->stream));
+    const size_t C = (size_t)x->g.C, rows = 4 * (size_t)x->g.R;
+    double* stage = copy_stage(x, dev);
+    launch_repitch(dev, (size_t)x->g.ld, stage, C, rows, C, x->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, stage, rows * C * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
 }

@@ -621,6 +621,22 @@ int dt_blocks(const Geom& g) {
 
 int march_rows() { return kMarchRows; }
 
+// A pitched 2-D copy on the device (host arrays in the reference layout <-> device rows).
+__global__ __launch_bounds__(256) void k_repitch(const double* __restrict__ src, size_t spitch,
+                                                 double* __restrict__ dst, size_t dpitch,
+                                                 size_t rows, size_t cols) {
+    const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= cols) return;
+    for (size_t r = blockIdx.y; r < rows; r += gridDim.y) dst[r * dpitch + j] = src[r * spitch + j];
+}
+
+void launch_repitch(const double* src, size_t spitch, double* dst, size_t dpitch, size_t rows,
+                    size_t cols, hipStream_t s) {
+    const unsigned gy = (unsigned)std::min<size_t>(rows, 16384);
+    hipLaunchKernelGGL(k_repitch, dim3((unsigned)((cols + 255) / 256), gy), dim3(256), 0, s, src,
+                       spitch, dst, dpitch, rows, cols);
+}
+
 void launch_dt_slots(const double* U, const Geom& g, const Phys& ph, const DtSlots& ds,
                      double* partial, double* dt_out, const double* clock, unsigned* faults,
                      hipStream_t s) {

@@ -205,5 +205,8 @@ __host__ __device__ inline int dt_slot_count(const Geom& g, int rows) {
     return ((g.R - 2 * GC + rows - 1) / rows) * ((g.C - 2 * GC + kStrip - 1) / kStrip);
 }
 int march_rows();  // rows a wavefront of the marching fused y sweep walks
+// rows x cols doubles from src (row pitch spitch) to dst (row pitch dpitch)
+void launch_repitch(const double* src, size_t spitch, double* dst, size_t dpitch, size_t rows,
+                    size_t cols, hipStream_t s);
 
 }  // namespace fvm
